@@ -1,0 +1,154 @@
+"""Benchmark of the hot path: the checkerboard Metropolis sweep at 4096^2.
+
+Workload: the plain monolayer at 4096 x 4096 sites (262 144 cells of side 8,
+u32 labels), contact J = 10, lambda_V = 2, V_t = 64, lambda_S = 1, S_t = 32,
+T = 10, Moore neighbourhood. One "step" is one Monte Carlo step (four colour
+passes, 4096^2 copy attempts) of one lattice; the timed steps go round-robin
+over four lattices seeded 1-4, so --steps 400 runs each seed for 100 MCS.
+Inputs are resident in HBM before the timed region starts.
+
+Prints one JSON line with site-attempts/s, MCS/s, the roofline of the sweep
+kernel (algorithmic bytes / measured launch time against 8 TB/s) and a CPU
+baseline (the serial oracle on this host, one thread, on a bounded sample).
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
+def cpu_baseline(side: int, seconds: float):
+    """The serial oracle timed on this host: one thread, whole MCS of the same
+    model at `side`^2 until about `seconds` have run."""
+    import oracle
+    from glazier_amd.workloads import plain_monolayer
+
+    model, labels, types = plain_monolayer(side, seed=1, surface=True)
+    sim = oracle.SerialSimulation(model, labels, types)
+    steps = 0
+    t0 = time.perf_counter()
+    while True:
+        sim.step(1)
+        steps += 1
+        el = time.perf_counter() - t0
+        if el >= seconds:
+            break
+    rate = steps * side * side / el
+    return {"value": rate, "unit": "site-attempts/s", "cores": 1, "kind": "port",
+            "sample": f"serial f64 oracle, {side}^2 plain monolayer with surface term, seed 1, "
+                      f"{steps} MCS in {el:.1f} s on one thread of {os.cpu_count()} "
+                      f"({cpu_model()})"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=400)
+    ap.add_argument("--warmup", type=int, default=8)
+    ap.add_argument("--side", type=int, default=4096)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported by this build",
+              file=sys.stderr)
+        return 2
+
+    import torch  # device memory plumbing only: the engine allocates its own
+    from glazier_amd import HipSimulation
+    from glazier_amd.workloads import plain_monolayer
+
+    side = args.side
+    sims = []
+    for seed in (1, 2, 3, 4):
+        model, labels, types = plain_monolayer(side, seed=seed, surface=True)
+        sims.append(HipSimulation(model, labels, types))
+    for i in range(args.warmup):
+        sims[i % 4].step(1)
+    torch.cuda.synchronize()
+    acc0 = [s.counters()[1] for s in sims]
+
+    per = [args.steps // 4 + (1 if i < args.steps % 4 else 0) for i in range(4)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for s, k in zip(sims, per):
+        if k:
+            s.step(k)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+
+    n = side * side
+    attempts = n * args.steps
+    accepted = sum(s.counters()[1] for s in sims) - sum(acc0)
+    a = accepted / attempts
+
+    # roofline of the sweep kernel: its launches timed with HIP events on the
+    # engine's own stream over a separate stretch of the same run
+    probe = sims[0]
+    probe.set_timing(True)
+    before_acc = probe.counters()[1]
+    probe.step(10)
+    probe.set_timing(False)
+    ms, launches = probe.kernel_time("sweep")
+    acc_probe = probe.counters()[1] - before_acc
+    avg_s = ms / launches / 1e3
+    bytes_per_launch = 4.0 * n + 4.0 * acc_probe / launches
+    achieved = bytes_per_launch / avg_s / 1e9
+
+    line = {
+        "metric": "site-attempts/s at 4096^2 (% HBM peak); batched vs looped replicas/s",
+        "value": attempts / elapsed,
+        "unit": "site-attempts/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32 labels, int64 counters, f32 energies",
+        "data": "synthetic",
+        "config": {"workload": f"plain monolayer {side}^2, 8x8 cells, J=10, lambda_V=2, V_t=64, "
+                               f"lambda_S=1, S_t=32, T=10, Moore; seeds 1-4 round robin",
+                   "side": side, "cells": (side // 8) ** 2, "seeds": [1, 2, 3, 4],
+                   "mcs_per_seed": per},
+        "mcs_per_s": args.steps / elapsed,
+        "acceptance": a,
+        "hbm_frac_algorithmic": attempts * (16.0 + 4.0 * a) / elapsed / (HBM_PEAK_GBS * 1e9),
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "sweep_kernel (one colour pass)",
+                     "avg_launch_us": avg_s * 1e6, "algorithmic_bytes_per_launch": bytes_per_launch},
+        "cpu_baseline": None,
+    }
+    for s in sims:
+        s.close()
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(side, args.cpu_seconds)
+    print(json.dumps(line))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

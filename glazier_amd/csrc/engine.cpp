@@ -1,0 +1,880 @@
+// engine.cpp - host driver of the MI355X Cellular Potts engine and its C ABI
+// (include/glazier_hip.h).
+//
+// Replaces GpuSimulation (glazier-cuda/src/lib.rs:749-1569): it owns the
+// device buffers, builds the model tables once, and enqueues each Monte
+// Carlo step on one HIP stream: the colour passes, the field step, activity
+// decay, and division/death, with no host synchronisation inside a call to
+// ghip_step. Kernels are precompiled for gfx950; nothing is compiled at run
+// time.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/glazier_hip.h"
+#include "ghip_device.h"
+
+using namespace ghip;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+ghip_status fail(ghip_status code, const std::string &msg) {
+    g_last_error = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                          \
+    do {                                                                                       \
+        hipError_t _e = (expr);                                                                \
+        if (_e != hipSuccess)                                                                  \
+            return fail(GHIP_DEVICE, std::string(#expr) + ": " + hipGetErrorString(_e));       \
+    } while (0)
+
+int ilog2(int v) {
+    int r = 0;
+    while ((1 << (r + 1)) <= v) ++r;
+    return r;
+}
+
+int pow2_floor(int v) { return v < 1 ? 1 : 1 << ilog2(v); }
+
+bool is_integer(double v) { return v == std::floor(v) && std::fabs(v) < 1048576.0; }
+
+struct TimedLaunch {
+    int family;
+    hipEvent_t start, stop;
+};
+
+const char *kFamilies[] = {"sweep", "fields", "moments", "population"};
+
+}  // namespace
+
+struct ghip_sim {
+    ghip_model m{};
+    int W = 0, H = 0, D = 0;
+    size_t N = 0;
+    bool d3 = false;
+    int colours = 4;
+    int n_off = 0;
+    int off[26][3] = {};
+    hipStream_t stream = nullptr;
+
+    uint32_t *labels[2] = {nullptr, nullptr};
+    int lab_cur = 0;
+    float *act[2] = {nullptr, nullptr};
+    int act_cur = 0;
+    float *field[GHIP_MAX_SPECIES][2] = {};
+    float *field_base[2] = {nullptr, nullptr};
+    int field_cur[GHIP_MAX_SPECIES] = {};
+    int64_t *A[2] = {nullptr, nullptr};
+    int64_t *Dl[3] = {nullptr, nullptr, nullptr};
+    int cw = 1;
+    uint32_t cap = 0;
+    int32_t *anchor = nullptr;
+    uint32_t *ltype = nullptr;
+    uint32_t *scratch = nullptr;
+    uint32_t *daughter = nullptr;
+    uint8_t *plan = nullptr;
+    double *geo = nullptr;
+    uint32_t *flags = nullptr;
+    uint32_t *type_copy = nullptr;
+    int32_t *anchor_copy = nullptr;
+    uint32_t *n_labels_dev = nullptr;
+    DeviceTables *tables = nullptr;
+    DeviceTables host_tables{};
+    unsigned long long *accepted = nullptr;
+
+    uint64_t pass = 0;
+    uint64_t mcs = 0;
+    uint64_t attempted = 0;
+    uint32_t sweep_flags = 0;
+    bool full = false;
+    bool has_pop = false, tracks_moments = false, has_mot = false;
+    int substeps[GHIP_MAX_SPECIES] = {};
+    float fd[GHIP_MAX_SPECIES] = {}, fk[GHIP_MAX_SPECIES] = {};
+    int medium_type = 0;
+
+    int tx_log2 = 6, ty_log2 = 3, tz_log2 = 0, threads = 512;
+    int tiles_x = 1, tiles_y = 1, tiles_z = 1, blocks = 1;
+    int fx_log2 = 6, fy_log2 = 5, fz = 1, f_tiles_x = 1, f_tiles_y = 1, f_blocks = 1, f_kmax = 4;
+
+    bool timing = false;
+    std::vector<TimedLaunch> pending;
+    std::vector<TimedLaunch> spare;
+    double ms[4] = {0, 0, 0, 0};
+    uint64_t launches[4] = {0, 0, 0, 0};
+
+    ~ghip_sim() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        void *bufs[] = {labels[0], labels[1], act[0], act[1], field_base[0], field_base[1], A[0],
+                        A[1], Dl[0], Dl[1], Dl[2], anchor, ltype, scratch, daughter, plan, geo,
+                        flags, type_copy, anchor_copy, n_labels_dev, tables, accepted};
+        for (void *p : bufs)
+            if (p) (void)hipFree(p);
+        for (auto *list : {&pending, &spare})
+            for (auto &t : *list) {
+                (void)hipEventDestroy(t.start);
+                (void)hipEventDestroy(t.stop);
+            }
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    CellArgs cell_args() {
+        CellArgs c{};
+        c.labels = labels[lab_cur];
+        c.state = A[pass % 2];
+        c.anchor = anchor;
+        c.n_labels = n_labels_dev;
+        c.ltype = ltype;
+        c.scratch_u32 = scratch;
+        c.daughter = daughter;
+        c.plan = plan;
+        c.geo = geo;
+        c.flags = flags;
+        c.tables = tables;
+        c.w = W;
+        c.h = H;
+        c.d = D;
+        c.cw = cw;
+        c.n_off = n_off;
+        std::memcpy(c.off, off, sizeof off);
+        c.capacity = cap;
+        return c;
+    }
+
+    hipError_t begin_timed(int family, TimedLaunch *out) {
+        if (!timing) return hipSuccess;
+        if (spare.empty()) {
+            TimedLaunch t{};
+            hipError_t e = hipEventCreate(&t.start);
+            if (e != hipSuccess) return e;
+            e = hipEventCreate(&t.stop);
+            if (e != hipSuccess) return e;
+            spare.push_back(t);
+        }
+        *out = spare.back();
+        spare.pop_back();
+        out->family = family;
+        return hipEventRecord(out->start, stream);
+    }
+
+    hipError_t end_timed(TimedLaunch &t) {
+        if (!timing) return hipSuccess;
+        hipError_t e = hipEventRecord(t.stop, stream);
+        pending.push_back(t);
+        return e;
+    }
+
+    hipError_t collect_timing() {
+        for (auto &t : pending) {
+            float elapsed = 0.0f;
+            hipError_t e = hipEventElapsedTime(&elapsed, t.start, t.stop);
+            if (e != hipSuccess) return e;
+            ms[t.family] += elapsed;
+            launches[t.family] += 1;
+            spare.push_back(t);
+        }
+        pending.clear();
+        return hipSuccess;
+    }
+};
+
+namespace {
+
+// The device tables from the model, in the device type index (type 0 is the
+// medium, pricing contact as the reference's cell_type[0]).
+void build_tables(ghip_sim *s) {
+    const ghip_model &m = s->m;
+    DeviceTables &t = s->host_tables;
+    std::memset(&t, 0, sizeof t);
+    const int nt = (int)m.n_types;
+    auto map = [&](int i) { return i == 0 ? s->medium_type : i; };
+    bool integral = true;
+    for (int a = 0; a < nt; ++a)
+        for (int b = 0; b < nt; ++b) {
+            double j = m.contact[map(a) * nt + map(b)];
+            t.J[a][b] = (float)j;
+        }
+    for (int a = 0; a < nt * nt; ++a)
+        if (!is_integer(m.contact[a])) integral = false;
+    bool conn = false, len = false, chem = false, mot = false, drift = false, pop = false;
+    for (int i = 0; i < nt; ++i) {
+        const ghip_cell_type &c = m.types[i];
+        t.lam_v[i] = (float)c.lambda_volume;
+        t.tgt_v[i] = (float)c.target_volume;
+        t.lam_s[i] = (float)c.lambda_surface;
+        t.tgt_s[i] = (float)c.target_surface;
+        if (!is_integer(c.lambda_volume) || !is_integer(c.target_volume) ||
+            !is_integer(c.lambda_surface) || !is_integer(c.target_surface))
+            integral = false;
+        t.max_act[i] = (float)c.max_activity;
+        t.coef_act[i] = c.max_activity > 0.0 ? (float)(c.lambda_activity / c.max_activity) : 0.0f;
+        for (int a = 0; a < 3; ++a) t.ext[i][a] = (float)c.external[a];
+        for (uint32_t k = 0; k < m.n_species; ++k) {
+            t.chi[i][k] = (float)m.chemotaxis[i][k];
+            t.sec[i][k] = (float)m.secretion[i][k];
+            t.up[i][k] = (float)m.uptake[i][k];
+            if (m.chemotaxis[i][k] != 0.0) chem = true;
+        }
+        t.lam_len[i] = c.lambda_length;
+        t.tgt_len[i] = c.target_length;
+        t.div_vol[i] = c.division_volume;
+        t.death[i] = c.death_rate;
+        t.connected[i] = c.connected ? 1u : 0u;
+        if (c.connected) conn = true;
+        if (c.lambda_length != 0.0) len = true;
+        if (c.lambda_activity != 0.0 && c.max_activity > 0.0) mot = true;
+        if (c.external[0] != 0.0 || c.external[1] != 0.0 || c.external[2] != 0.0) drift = true;
+        if (c.division_volume > 0.0 || c.death_rate > 0.0) pop = true;
+    }
+    if (len || chem || mot || drift) integral = false;
+    const double lim = m.temperature * 24.0 * std::log(2.0);
+    if (integral && lim + 1.0 < (double)kMaxThr) {
+        t.n_thr = (uint32_t)std::floor(lim) + 1;
+        t.thr[0] = 1u << 24;
+        for (uint32_t i = 1; i < t.n_thr; ++i)
+            t.thr[i] = (uint32_t)std::ceil(16777216.0 * std::exp(-(double)i / m.temperature));
+    } else {
+        integral = false;
+        t.n_thr = 0;
+    }
+    t.accept_c = (float)(1.4426950408889634 / m.temperature);
+    // connectivity: one-piece table of the planar ring, adjacency of the volume ring
+    int pos[26][3], k = 0;
+    for (int dz = -1; dz <= 1; ++dz)
+        for (int dy = -1; dy <= 1; ++dy)
+            for (int dx = -1; dx <= 1; ++dx) {
+                if (!dx && !dy && !dz) continue;
+                pos[k][0] = dx;
+                pos[k][1] = dy;
+                pos[k][2] = dz;
+                ++k;
+            }
+    for (int a = 0; a < 26; ++a) {
+        uint32_t mask = 0;
+        for (int b = 0; b < 26; ++b)
+            if (std::abs(pos[a][0] - pos[b][0]) <= 1 && std::abs(pos[a][1] - pos[b][1]) <= 1 &&
+                std::abs(pos[a][2] - pos[b][2]) <= 1)
+                mask |= 1u << b;
+        t.adj3d[a] = mask;
+    }
+    int ppos[8][2], q = 0;
+    for (int dy = -1; dy <= 1; ++dy)
+        for (int dx = -1; dx <= 1; ++dx) {
+            if (!dx && !dy) continue;
+            ppos[q][0] = dx;
+            ppos[q][1] = dy;
+            ++q;
+        }
+    for (uint32_t present = 0; present < 256; ++present) {
+        bool one = true;
+        if (present && (present & (present - 1))) {
+            uint32_t reached = present & (~present + 1u);
+            for (;;) {
+                uint32_t grown = reached;
+                for (int a = 0; a < 8; ++a) {
+                    if (!(reached & (1u << a))) continue;
+                    for (int b = 0; b < 8; ++b)
+                        if ((present & (1u << b)) && std::abs(ppos[a][0] - ppos[b][0]) <= 1 &&
+                            std::abs(ppos[a][1] - ppos[b][1]) <= 1)
+                            grown |= 1u << b;
+                }
+                if (grown == reached) break;
+                reached = grown;
+            }
+            one = reached == present;
+        }
+        if (one) t.piece2d[present >> 5] |= 1u << (present & 31);
+    }
+    s->sweep_flags = 0;
+    if (integral) s->sweep_flags |= kIntegral;
+    if (conn) s->sweep_flags |= kConnected;
+    if (len) s->sweep_flags |= kLength;
+    if (chem) s->sweep_flags |= kChemotaxis;
+    if (mot) s->sweep_flags |= kMotility;
+    if (drift) s->sweep_flags |= kDrift;
+    s->has_pop = pop;
+    s->has_mot = mot;
+    s->tracks_moments = len || pop;
+    if (s->tracks_moments) s->sweep_flags |= kMoments;
+    s->full = conn || len || chem || mot || drift || s->tracks_moments;
+}
+
+int substeps_for(const ghip_model &m, uint32_t k) {
+    const ghip_species &sp = m.species[k];
+    if (sp.substeps > 0) return sp.substeps;
+    if (sp.diffusion <= 0.0) return 1;
+    double dims = m.depth > 1 ? 3.0 : 2.0;
+    double c = std::ceil(sp.diffusion / (1.0 / (2.0 * dims)));
+    return c < 1.0 ? 1 : (int)c;
+}
+
+// Tile shapes: a 2D pass runs 64 x 8 targets (128 x 16 sites, 512 lanes) and
+// shrinks to the lattice; a 3D pass runs 16 x 4 x 4 targets.
+void choose_tiles(ghip_sim *s) {
+    if (!s->d3) {
+        int tx = std::min(64, pow2_floor(s->W / 2));
+        int ty = std::min(pow2_floor(512 / tx), pow2_floor(s->H / 2));
+        s->tx_log2 = ilog2(tx);
+        s->ty_log2 = ilog2(ty);
+        s->tz_log2 = 0;
+        s->threads = std::max(64, tx * ty);
+        s->tiles_x = (s->W / 2 + tx - 1) / tx;
+        s->tiles_y = (s->H / 2 + ty - 1) / ty;
+        s->tiles_z = 1;
+    } else {
+        int tx = std::min(16, pow2_floor(s->W / 2));
+        int ty = std::min(4, pow2_floor(s->H / 2));
+        int tz = std::min(pow2_floor(256 / (tx * ty)), pow2_floor(s->D / 2));
+        s->tx_log2 = ilog2(tx);
+        s->ty_log2 = ilog2(ty);
+        s->tz_log2 = ilog2(tz);
+        s->threads = std::max(64, tx * ty * tz);
+        s->tiles_x = (s->W / 2 + tx - 1) / tx;
+        s->tiles_y = (s->H / 2 + ty - 1) / ty;
+        s->tiles_z = (s->D / 2 + tz - 1) / tz;
+    }
+    s->threads = ((s->threads + 63) / 64) * 64;
+    s->blocks = s->tiles_x * s->tiles_y * s->tiles_z;
+    // fields: 64 x 32 interior in a plane, 16 x 8 x 8 in a volume
+    if (!s->d3) {
+        s->fx_log2 = ilog2(std::min(64, pow2_floor(s->W)));
+        s->fy_log2 = ilog2(std::min(32, pow2_floor(s->H)));
+        s->fz = 1;
+        s->f_kmax = 4;
+    } else {
+        s->fx_log2 = ilog2(std::min(16, pow2_floor(s->W)));
+        s->fy_log2 = ilog2(std::min(8, pow2_floor(s->H)));
+        s->fz = std::min(8, pow2_floor(s->D));
+        s->f_kmax = 2;
+    }
+    s->f_tiles_x = (s->W + (1 << s->fx_log2) - 1) >> s->fx_log2;
+    s->f_tiles_y = (s->H + (1 << s->fy_log2) - 1) >> s->fy_log2;
+    int f_tiles_z = s->d3 ? (s->D + s->fz - 1) / s->fz : 1;
+    s->f_blocks = s->f_tiles_x * s->f_tiles_y * f_tiles_z;
+}
+
+ghip_status run_sweep_pass(ghip_sim *s, int colour) {
+    SweepArgs a{};
+    const uint64_t p = s->pass;
+    const bool mot = s->has_mot;
+    if (mot) {
+        a.labels_prev = s->labels[s->lab_cur];
+        a.labels = s->labels[1 - s->lab_cur];
+        a.act_prev = s->act[s->act_cur];
+        a.act = s->act[1 - s->act_cur];
+    } else {
+        a.labels = s->labels[s->lab_cur];
+        a.labels_prev = a.labels;
+        a.act = s->act[s->act_cur];
+        a.act_prev = a.act;
+    }
+    for (uint32_t k = 0; k < s->m.n_species; ++k) a.fields[k] = s->field[k][s->field_cur[k]];
+    a.a_cur = s->A[p % 2];
+    a.e_prev = s->Dl[(p + 2) % 3];
+    a.f_acc = s->Dl[p % 3];
+    a.a_next = s->A[(p + 1) % 2];
+    a.g_zero = s->Dl[(p + 1) % 3];
+    a.anchor = s->anchor;
+    a.tables = s->tables;
+    a.n_labels = s->n_labels_dev;
+    a.accepted = s->accepted;
+    a.w = s->W;
+    a.h = s->H;
+    a.d = s->D;
+    a.cx = colour & 1;
+    a.cy = (colour >> 1) & 1;
+    a.cz = (colour >> 2) & 1;
+    a.key = pass_key(s->m.seed, s->m.replica, s->mcs, (uint32_t)colour);
+    a.key2 = second_key(a.key);
+    a.tx_log2 = s->tx_log2;
+    a.ty_log2 = s->ty_log2;
+    a.tz_log2 = s->tz_log2;
+    a.halo = mot ? 2 : 1;
+    a.n_off = s->n_off;
+    a.n_types = s->m.n_types;
+    a.n_thr = s->host_tables.n_thr;
+    a.cell_words = s->cw;
+    a.n_species = (int)s->m.n_species;
+    a.flags = s->sweep_flags;
+    const uint64_t words = (uint64_t)s->cap * s->cw;
+    a.fold_per_block = (uint32_t)((words + s->blocks - 1) / s->blocks);
+    a.xcd_chunk = (s->blocks % 8 == 0) ? s->blocks / 8 : 0;
+    a.tiles_x = s->tiles_x;
+    a.tiles_y = s->tiles_y;
+    a.tiles_z = s->tiles_z;
+    std::memcpy(a.off, s->off, sizeof s->off);
+    TimedLaunch t{};
+    HIP_TRY(s->begin_timed(0, &t));
+    HIP_TRY(launch_sweep(a, s->d3, s->full, s->blocks, s->threads, s->stream));
+    HIP_TRY(s->end_timed(t));
+    s->pass += 1;
+    if (mot) {
+        s->lab_cur = 1 - s->lab_cur;
+        s->act_cur = 1 - s->act_cur;
+    }
+    return GHIP_OK;
+}
+
+ghip_status run_fields(ghip_sim *s) {
+    TimedLaunch t{};
+    HIP_TRY(s->begin_timed(1, &t));
+    for (uint32_t k = 0; k < s->m.n_species; ++k) {
+        int left = s->substeps[k];
+        while (left > 0) {
+            int take = std::min(left, s->f_kmax);
+            left -= take;
+            FieldArgs f{};
+            f.in = s->field[k][s->field_cur[k]];
+            f.out = s->field[k][1 - s->field_cur[k]];
+            f.labels = s->labels[s->lab_cur];
+            f.tables = s->tables;
+            f.w = s->W;
+            f.h = s->H;
+            f.d = s->D;
+            f.ix_log2 = s->fx_log2;
+            f.iy_log2 = s->fy_log2;
+            f.iz = s->fz;
+            f.steps = take;
+            f.exchange = left == 0 ? 1 : 0;
+            f.species = (int)k;
+            f.dcoef = s->fd[k];
+            f.kcoef = s->fk[k];
+            f.tiles_x = s->f_tiles_x;
+            f.tiles_y = s->f_tiles_y;
+            HIP_TRY(launch_field(f, s->d3, s->f_blocks, s->stream));
+            s->field_cur[k] = 1 - s->field_cur[k];
+        }
+    }
+    HIP_TRY(s->end_timed(t));
+    return GHIP_OK;
+}
+
+ghip_status run_cells(ghip_sim *s) {
+    // bring the per-label state current: A[p%2] += D[(p+2)%3]
+    const uint64_t p = s->pass;
+    HIP_TRY(launch_consolidate(s->A[p % 2], s->Dl[(p + 2) % 3], s->n_labels_dev, s->cw, s->cap,
+                               s->stream));
+    TimedLaunch t{};
+    HIP_TRY(s->begin_timed(3, &t));
+    HIP_TRY(launch_fill_u32(s->flags, 4, 0u, s->stream));
+    CellArgs c = s->cell_args();
+    if (s->has_pop) {
+        c.key = pass_key(s->m.seed, s->m.replica, s->mcs, 0xDEADCE11U);
+        c.key2 = second_key(c.key);
+        HIP_TRY(launch_population(c, s->N, s->type_copy, s->anchor_copy, s->stream));
+    }
+    HIP_TRY(launch_recentre_check(c, s->N, s->stream));
+    HIP_TRY(s->end_timed(t));
+    return GHIP_OK;
+}
+
+ghip_status read_state(const ghip_sim *s, std::vector<int64_t> &cur, uint32_t &n_labels) {
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    HIP_TRY(hipMemcpy(&n_labels, s->n_labels_dev, 4, hipMemcpyDeviceToHost));
+    const uint64_t p = s->pass;
+    size_t words = (size_t)n_labels * s->cw;
+    std::vector<int64_t> e(words);
+    cur.resize(words);
+    if (words) {
+        HIP_TRY(hipMemcpy(cur.data(), s->A[p % 2], words * 8, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(e.data(), s->Dl[(p + 2) % 3], words * 8, hipMemcpyDeviceToHost));
+    }
+    for (size_t i = 0; i < words; ++i) cur[i] += e[i];
+    return GHIP_OK;
+}
+
+ghip_status read_labels_raw(const ghip_sim *s, std::vector<uint32_t> &out) {
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    out.resize(s->N);
+    HIP_TRY(hipMemcpy(out.data(), s->labels[s->lab_cur], s->N * 4, hipMemcpyDeviceToHost));
+    return GHIP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *ghip_last_error(void) { return g_last_error.c_str(); }
+
+ghip_status ghip_create(const ghip_model *model, const uint32_t *labels, size_t n_sites,
+                        const uint8_t *cell_type, size_t n_labels, const double *fields,
+                        const double *activity, ghip_sim **out) {
+    g_last_error.clear();
+    if (!model || !labels || !cell_type || !out) return fail(GHIP_BAD_ARG, "a required pointer is null");
+    *out = nullptr;
+    const ghip_model &m = *model;
+    if (m.width == 0 || m.height == 0 || m.depth == 0)
+        return fail(GHIP_BAD_ARG, "the lattice has a zero dimension");
+    if (m.width % 2 || m.height % 2 || (m.depth > 1 && m.depth % 2)) {
+        char buf[160];
+        std::snprintf(buf, sizeof buf,
+                      "the checkerboard split needs even sides; the lattice is %u by %u by %u",
+                      m.width, m.height, m.depth);
+        return fail(GHIP_ODD_LATTICE, buf);
+    }
+    if (m.has_nematic)
+        return fail(GHIP_UNSUPPORTED, "the nematic coupling is not priced by the hip engine; "
+                                      "refusing rather than dropping it");
+    if (m.n_types < 1 || m.n_types > GHIP_MAX_TYPES)
+        return fail(GHIP_UNSUPPORTED, "the hip engine takes 1 to 16 cell types");
+    if (m.n_species > GHIP_MAX_SPECIES) return fail(GHIP_UNSUPPORTED, "the hip engine takes up to 4 species");
+    if (m.neighbour_order < 1 || m.neighbour_order > 3)
+        return fail(GHIP_BAD_ARG, "neighbour order is outside the range one to three");
+    if (!(m.temperature > 0.0)) return fail(GHIP_BAD_ARG, "the temperature must be positive");
+    const size_t N = (size_t)m.width * m.height * m.depth;
+    if (n_sites != N) return fail(GHIP_BAD_ARG, "the label field does not match the lattice");
+    if (N >= 0xFFFFFFFFull) return fail(GHIP_UNSUPPORTED, "the lattice exceeds 2^32 sites");
+    if (n_labels < 1 || n_labels >= (size_t)kIdMask) return fail(GHIP_BAD_ARG, "label count out of range");
+    for (size_t l = 1; l < n_labels; ++l)
+        if (cell_type[l] < 1 || cell_type[l] >= m.n_types)
+            return fail(GHIP_BAD_ARG, "every cell needs a type between 1 and n_types - 1");
+    if (cell_type[0] >= m.n_types) return fail(GHIP_BAD_ARG, "the medium's type is out of range");
+    for (size_t i = 0; i < N; ++i)
+        if (labels[i] >= n_labels) return fail(GHIP_BAD_ARG, "a site holds a label beyond n_labels");
+
+    int dev_count = 0;
+    if (hipGetDeviceCount(&dev_count) != hipSuccess || dev_count == 0)
+        return fail(GHIP_DEVICE, "no HIP device is available");
+
+    ghip_sim *s = new ghip_sim();
+    s->m = m;
+    s->W = (int)m.width;
+    s->H = (int)m.height;
+    s->D = (int)m.depth;
+    s->N = N;
+    s->d3 = m.depth > 1;
+    s->colours = s->d3 ? 8 : 4;
+    s->medium_type = cell_type[0];
+    // neighbour offsets (glazier-core/src/lattice.rs:69-88)
+    {
+        static const int o1[6][3] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
+        static const int o2[18][3] = {{1, 0, 0},  {-1, 0, 0}, {0, 1, 0},   {0, -1, 0},  {0, 0, 1},
+                                      {0, 0, -1}, {1, 1, 0},  {1, -1, 0},  {-1, 1, 0},  {-1, -1, 0},
+                                      {1, 0, 1},  {1, 0, -1}, {-1, 0, 1},  {-1, 0, -1}, {0, 1, 1},
+                                      {0, 1, -1}, {0, -1, 1}, {0, -1, -1}};
+        int k = 0;
+        if (s->d3 && m.neighbour_order == 1) {
+            for (int i = 0; i < 6; ++i, ++k) std::memcpy(s->off[k], o1[i], sizeof o1[i]);
+        } else if (s->d3 && m.neighbour_order == 2) {
+            for (int i = 0; i < 18; ++i, ++k) std::memcpy(s->off[k], o2[i], sizeof o2[i]);
+        } else if (!s->d3 && m.neighbour_order == 1) {
+            for (int i = 0; i < 4; ++i, ++k) std::memcpy(s->off[k], o1[i], sizeof o1[i]);
+        } else {
+            for (int dz = -1; dz <= 1; ++dz)
+                for (int dy = -1; dy <= 1; ++dy)
+                    for (int dx = -1; dx <= 1; ++dx) {
+                        if (!dx && !dy && !dz) continue;
+                        if (!s->d3 && dz) continue;
+                        s->off[k][0] = dx;
+                        s->off[k][1] = dy;
+                        s->off[k][2] = dz;
+                        ++k;
+                    }
+        }
+        s->n_off = k;
+    }
+    build_tables(s);
+    choose_tiles(s);
+    s->cw = s->tracks_moments ? 10 : 1;
+    uint64_t cap = n_labels;
+    if (s->has_pop) cap = std::max<uint64_t>(2 * n_labels + 1024, n_labels + N / 16);
+    cap = std::min<uint64_t>(cap, kIdMask - 1);
+    s->cap = (uint32_t)cap;
+    for (uint32_t k = 0; k < m.n_species; ++k) {
+        s->substeps[k] = substeps_for(m, k);
+        s->fd[k] = (float)(m.species[k].diffusion / (double)s->substeps[k]);
+        s->fk[k] = (float)(m.species[k].decay / (double)s->substeps[k]);
+    }
+
+    auto cleanup = [&](ghip_status st) {
+        delete s;
+        return st;
+    };
+#define CREATE_TRY(expr)                                                                       \
+    do {                                                                                       \
+        hipError_t _e = (expr);                                                                \
+        if (_e != hipSuccess)                                                                  \
+            return cleanup(fail(GHIP_DEVICE, std::string(#expr) + ": " + hipGetErrorString(_e))); \
+    } while (0)
+
+    CREATE_TRY(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+    CREATE_TRY(hipMalloc(&s->labels[0], N * 4));
+    if (s->has_mot) CREATE_TRY(hipMalloc(&s->labels[1], N * 4));
+    CREATE_TRY(hipMalloc(&s->act[0], N * 4));
+    if (s->has_mot) CREATE_TRY(hipMalloc(&s->act[1], N * 4));
+    // species-major halves; each species ping-pongs between its two slices
+    if (m.n_species) {
+        CREATE_TRY(hipMalloc(&s->field_base[0], (size_t)m.n_species * N * 4));
+        CREATE_TRY(hipMalloc(&s->field_base[1], (size_t)m.n_species * N * 4));
+        for (uint32_t k = 0; k < m.n_species; ++k) {
+            s->field[k][0] = s->field_base[0] + k * N;
+            s->field[k][1] = s->field_base[1] + k * N;
+        }
+    }
+    const size_t words = (size_t)cap * s->cw;
+    for (int i = 0; i < 2; ++i) CREATE_TRY(hipMalloc(&s->A[i], words * 8));
+    for (int i = 0; i < 3; ++i) CREATE_TRY(hipMalloc(&s->Dl[i], words * 8));
+    CREATE_TRY(hipMalloc(&s->anchor, (size_t)cap * 3 * 4));
+    CREATE_TRY(hipMalloc(&s->ltype, (size_t)cap * 4));
+    CREATE_TRY(hipMalloc(&s->scratch, (size_t)cap * 4));
+    CREATE_TRY(hipMalloc(&s->daughter, (size_t)cap * 4));
+    CREATE_TRY(hipMalloc(&s->plan, (size_t)cap));
+    CREATE_TRY(hipMalloc(&s->geo, (size_t)cap * 6 * 8));
+    CREATE_TRY(hipMalloc(&s->flags, 16));
+    CREATE_TRY(hipMalloc(&s->type_copy, (size_t)cap * 4));
+    CREATE_TRY(hipMalloc(&s->anchor_copy, (size_t)cap * 3 * 4));
+    CREATE_TRY(hipMalloc(&s->n_labels_dev, 4));
+    CREATE_TRY(hipMalloc(&s->tables, sizeof(DeviceTables)));
+    CREATE_TRY(hipMalloc(&s->accepted, kAcceptSlots * 8));
+
+    // uploads: packed label words, per-label type words, fields, activity
+    std::vector<uint32_t> packed(N);
+    for (size_t i = 0; i < N; ++i) {
+        uint32_t l = labels[i];
+        packed[i] = l == 0 ? 0u : (l | ((uint32_t)cell_type[l] << kTypeShift));
+    }
+    std::vector<uint32_t> lt(cap, 0u);
+    for (size_t l = 1; l < n_labels; ++l) lt[l] = (uint32_t)l | ((uint32_t)cell_type[l] << kTypeShift);
+    CREATE_TRY(hipMemcpy(s->labels[0], packed.data(), N * 4, hipMemcpyHostToDevice));
+    if (s->has_mot) CREATE_TRY(hipMemcpy(s->labels[1], packed.data(), N * 4, hipMemcpyHostToDevice));
+    CREATE_TRY(hipMemcpy(s->ltype, lt.data(), (size_t)cap * 4, hipMemcpyHostToDevice));
+    {
+        std::vector<float> a(N, 0.0f);
+        if (activity)
+            for (size_t i = 0; i < N; ++i) a[i] = (float)activity[i];
+        CREATE_TRY(hipMemcpy(s->act[0], a.data(), N * 4, hipMemcpyHostToDevice));
+        if (s->has_mot) CREATE_TRY(hipMemcpy(s->act[1], a.data(), N * 4, hipMemcpyHostToDevice));
+    }
+    for (uint32_t k = 0; k < m.n_species; ++k) {
+        std::vector<float> f(N);
+        for (size_t i = 0; i < N; ++i) f[i] = (float)(fields ? fields[k * N + i] : m.species[k].initial);
+        CREATE_TRY(hipMemcpy(s->field[k][0], f.data(), N * 4, hipMemcpyHostToDevice));
+    }
+    uint32_t nl = (uint32_t)n_labels;
+    CREATE_TRY(hipMemcpy(s->n_labels_dev, &nl, 4, hipMemcpyHostToDevice));
+    CREATE_TRY(hipMemcpy(s->tables, &s->host_tables, sizeof(DeviceTables), hipMemcpyHostToDevice));
+    CREATE_TRY(hipMemset(s->accepted, 0, kAcceptSlots * 8));
+    CREATE_TRY(hipMemset(s->flags, 0, 16));
+    for (int i = 0; i < 3; ++i) CREATE_TRY(hipMemset(s->Dl[i], 0, words * 8));
+    CREATE_TRY(hipMemset(s->A[1], 0, words * 8));
+    CREATE_TRY(hipMemset(s->anchor, 0, (size_t)cap * 12));
+    CREATE_TRY(hipDeviceSynchronize());
+
+    // initial counters and anchors
+    CellArgs c = s->cell_args();
+    CREATE_TRY(launch_recount(c, N, s->stream));
+    if (s->tracks_moments) CREATE_TRY(launch_initial_anchors(c, N, s->stream));
+    CREATE_TRY(hipStreamSynchronize(s->stream));
+    *out = s;
+    return GHIP_OK;
+#undef CREATE_TRY
+}
+
+ghip_status ghip_destroy(ghip_sim *sim) {
+    delete sim;
+    return GHIP_OK;
+}
+
+ghip_status ghip_step(ghip_sim *s, uint64_t steps) {
+    if (!s) return fail(GHIP_BAD_ARG, "null handle");
+    for (uint64_t q = 0; q < steps; ++q) {
+        int order[8];
+        colour_order(s->m.seed, s->m.replica, s->mcs, s->colours, order);
+        for (int c = 0; c < s->colours; ++c) {
+            ghip_status st = run_sweep_pass(s, order[c]);
+            if (st != GHIP_OK) return st;
+        }
+        if (s->m.n_species) {
+            ghip_status st = run_fields(s);
+            if (st != GHIP_OK) return st;
+        }
+        if (s->has_mot) HIP_TRY(launch_activity_decay(s->act[s->act_cur], s->N, s->stream));
+        if (s->tracks_moments) {
+            ghip_status st = run_cells(s);
+            if (st != GHIP_OK) return st;
+        }
+        s->mcs += 1;
+        s->attempted += s->N;
+    }
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    if (s->timing) HIP_TRY(s->collect_timing());
+    if (s->has_pop) {
+        uint32_t fl[4];
+        HIP_TRY(hipMemcpy(fl, s->flags, 16, hipMemcpyDeviceToHost));
+        if (fl[2]) return fail(GHIP_DEVICE, "the label capacity ran out; divisions were dropped");
+    }
+    return GHIP_OK;
+}
+
+uint64_t ghip_mcs(const ghip_sim *s) { return s ? s->mcs : 0; }
+
+ghip_status ghip_n_labels(const ghip_sim *s, size_t *out) {
+    if (!s || !out) return fail(GHIP_BAD_ARG, "null argument");
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    uint32_t n = 0;
+    HIP_TRY(hipMemcpy(&n, s->n_labels_dev, 4, hipMemcpyDeviceToHost));
+    *out = n;
+    return GHIP_OK;
+}
+
+ghip_status ghip_labels(const ghip_sim *s, uint32_t *out, size_t n) {
+    if (!s || !out) return fail(GHIP_BAD_ARG, "null argument");
+    if (n != s->N) return fail(GHIP_BAD_ARG, "the buffer does not match the lattice");
+    std::vector<uint32_t> raw;
+    ghip_status st = read_labels_raw(s, raw);
+    if (st != GHIP_OK) return st;
+    for (size_t i = 0; i < n; ++i) out[i] = label_id(raw[i]);
+    return GHIP_OK;
+}
+
+ghip_status ghip_volumes(const ghip_sim *s, uint32_t *out, size_t n) {
+    if (!s || !out) return fail(GHIP_BAD_ARG, "null argument");
+    std::vector<int64_t> cur;
+    uint32_t nl = 0;
+    ghip_status st = read_state(s, cur, nl);
+    if (st != GHIP_OK) return st;
+    if (n != nl) return fail(GHIP_BAD_ARG, "the buffer does not match the label count");
+    uint64_t cells = 0;
+    for (size_t l = 1; l < nl; ++l) {
+        int64_t v = unpack_vol(cur[l * s->cw]);
+        out[l] = (uint32_t)(v > 0 ? v : 0);
+        cells += out[l];
+    }
+    out[0] = (uint32_t)(s->N - cells);
+    return GHIP_OK;
+}
+
+ghip_status ghip_surfaces(const ghip_sim *s, int64_t *out, size_t n) {
+    if (!s || !out) return fail(GHIP_BAD_ARG, "null argument");
+    std::vector<int64_t> cur;
+    uint32_t nl = 0;
+    ghip_status st = read_state(s, cur, nl);
+    if (st != GHIP_OK) return st;
+    if (n != nl) return fail(GHIP_BAD_ARG, "the buffer does not match the label count");
+    for (size_t l = 1; l < nl; ++l) out[l] = unpack_surf(cur[l * s->cw]);
+    // the medium is not tracked during a pass: count it from the lattice
+    std::vector<uint32_t> raw;
+    st = read_labels_raw(s, raw);
+    if (st != GHIP_OK) return st;
+    int64_t s0 = 0;
+    const size_t plane = (size_t)s->W * s->H;
+    for (size_t i = 0; i < s->N; ++i) {
+        if (raw[i] != 0) continue;
+        int z = (int)(i / plane), y = (int)((i % plane) / s->W), x = (int)(i % s->W);
+        for (int k = 0; k < s->n_off; ++k) {
+            int nx = (x + s->off[k][0] + s->W) % s->W, ny = (y + s->off[k][1] + s->H) % s->H;
+            int nz = (z + s->off[k][2] + s->D) % s->D;
+            size_t nb = ((size_t)nz * s->H + ny) * s->W + nx;
+            if (nb != i && raw[nb] != 0) ++s0;
+        }
+    }
+    out[0] = s0;
+    return GHIP_OK;
+}
+
+ghip_status ghip_cell_types(const ghip_sim *s, uint8_t *out, size_t n) {
+    if (!s || !out) return fail(GHIP_BAD_ARG, "null argument");
+    size_t nl = 0;
+    ghip_status st = ghip_n_labels(s, &nl);
+    if (st != GHIP_OK) return st;
+    if (n != nl) return fail(GHIP_BAD_ARG, "the buffer does not match the label count");
+    std::vector<uint32_t> lt(nl);
+    HIP_TRY(hipMemcpy(lt.data(), s->ltype, nl * 4, hipMemcpyDeviceToHost));
+    out[0] = (uint8_t)s->medium_type;
+    for (size_t l = 1; l < nl; ++l) out[l] = (uint8_t)label_type(lt[l]);
+    return GHIP_OK;
+}
+
+ghip_status ghip_field_f32(const ghip_sim *s, uint32_t species, float *out, size_t n) {
+    if (!s || !out) return fail(GHIP_BAD_ARG, "null argument");
+    if (species >= s->m.n_species) return fail(GHIP_BAD_ARG, "no such species");
+    if (n != s->N) return fail(GHIP_BAD_ARG, "the buffer does not match the lattice");
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    HIP_TRY(hipMemcpy(out, s->field[species][s->field_cur[species]], n * 4, hipMemcpyDeviceToHost));
+    return GHIP_OK;
+}
+
+ghip_status ghip_field(const ghip_sim *s, uint32_t species, double *out, size_t n) {
+    std::vector<float> f(n);
+    ghip_status st = ghip_field_f32(s, species, f.data(), n);
+    if (st != GHIP_OK) return st;
+    for (size_t i = 0; i < n; ++i) out[i] = (double)f[i];
+    return GHIP_OK;
+}
+
+ghip_status ghip_set_field(ghip_sim *s, uint32_t species, const double *values, size_t n) {
+    if (!s || !values) return fail(GHIP_BAD_ARG, "null argument");
+    if (species >= s->m.n_species) return fail(GHIP_BAD_ARG, "no such species");
+    if (n != s->N) return fail(GHIP_BAD_ARG, "the field does not match the lattice");
+    std::vector<float> f(n);
+    for (size_t i = 0; i < n; ++i) f[i] = (float)values[i];
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    HIP_TRY(hipMemcpy(s->field[species][s->field_cur[species]], f.data(), n * 4, hipMemcpyHostToDevice));
+    return GHIP_OK;
+}
+
+ghip_status ghip_activity_f32(const ghip_sim *s, float *out, size_t n) {
+    if (!s || !out) return fail(GHIP_BAD_ARG, "null argument");
+    if (n != s->N) return fail(GHIP_BAD_ARG, "the buffer does not match the lattice");
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    HIP_TRY(hipMemcpy(out, s->act[s->act_cur], n * 4, hipMemcpyDeviceToHost));
+    return GHIP_OK;
+}
+
+ghip_status ghip_counters(const ghip_sim *s, uint64_t *attempted, uint64_t *accepted) {
+    if (!s || !attempted || !accepted) return fail(GHIP_BAD_ARG, "null argument");
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    unsigned long long slots[kAcceptSlots];
+    HIP_TRY(hipMemcpy(slots, s->accepted, sizeof slots, hipMemcpyDeviceToHost));
+    uint64_t total = 0;
+    for (int i = 0; i < kAcceptSlots; ++i) total += slots[i];
+    *attempted = s->attempted;
+    *accepted = total;
+    return GHIP_OK;
+}
+
+ghip_status ghip_cell_state(const ghip_sim *s, int64_t *out, size_t n, uint32_t *words_per_label) {
+    if (!s || !words_per_label) return fail(GHIP_BAD_ARG, "null argument");
+    const uint32_t words = s->tracks_moments ? 11 : 2;
+    *words_per_label = words;
+    if (!out) return GHIP_OK;
+    std::vector<int64_t> cur;
+    uint32_t nl = 0;
+    ghip_status st = read_state(s, cur, nl);
+    if (st != GHIP_OK) return st;
+    if (n != (size_t)nl * words) return fail(GHIP_BAD_ARG, "the buffer does not match the label count");
+    for (size_t l = 0; l < nl; ++l) {
+        const int64_t *c = cur.data() + l * s->cw;
+        out[words * l] = unpack_vol(c[0]);
+        out[words * l + 1] = unpack_surf(c[0]);
+        for (uint32_t q = 2; q < words; ++q) out[words * l + q] = c[q - 1];
+    }
+    return GHIP_OK;
+}
+
+ghip_status ghip_set_timing(ghip_sim *s, int on) {
+    if (!s) return fail(GHIP_BAD_ARG, "null handle");
+    s->timing = on != 0;
+    return GHIP_OK;
+}
+
+ghip_status ghip_kernel_time(const ghip_sim *s, const char *family, double *total_ms, uint64_t *launches) {
+    if (!s || !family || !total_ms || !launches) return fail(GHIP_BAD_ARG, "null argument");
+    for (int i = 0; i < 4; ++i)
+        if (std::strcmp(family, kFamilies[i]) == 0) {
+            *total_ms = s->ms[i];
+            *launches = s->launches[i];
+            return GHIP_OK;
+        }
+    return fail(GHIP_BAD_ARG, "unknown kernel family");
+}
+
+}  // extern "C"

@@ -1,0 +1,137 @@
+// ghip_common.h - arithmetic shared by the host driver and the gfx950
+// kernels of the Cellular Potts engine: the counter hash, the draw layout,
+// the f32 exponential and logarithm, and the periodic folds.
+//
+// Every function here is a fixed sequence of IEEE f32/f64 or integer
+// operations (the library is built with -ffp-contract=off), so a kernel and
+// the host code that checks it round identically.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define GHD __host__ __device__ __forceinline__
+
+namespace ghip {
+
+// A 32-bit integer finaliser: two xor-shift-multiply rounds (Wellons,
+// "hash-prospector", lowbias32). A bijection on 32 bits.
+GHD uint32_t hash32(uint32_t x) {
+    x ^= x >> 16;
+    x *= 0x7feb352dU;
+    x ^= x >> 15;
+    x *= 0x846ca68bU;
+    x ^= x >> 16;
+    return x;
+}
+
+// Key of one colour pass, folding seed, replica, step and colour in turn.
+// The driver computes it once per launch; the kernel only mixes in the site.
+GHD uint32_t pass_key(uint64_t seed, uint32_t replica, uint64_t step, uint32_t colour) {
+    uint32_t k = hash32((uint32_t)seed ^ 0x243F6A88U);
+    k = hash32(k ^ (uint32_t)(seed >> 32));
+    k = hash32(k ^ replica);
+    k = hash32(k ^ (uint32_t)step);
+    k = hash32(k ^ (uint32_t)(step >> 32));
+    k = hash32(k ^ colour);
+    return k;
+}
+
+GHD uint32_t second_key(uint32_t key) { return hash32(key ^ 0x85EBCA6BU); }
+
+// The one draw a target takes in a pass: a bijection of the site for a fixed
+// pass, so no two targets of a pass share a draw.
+GHD uint32_t site_draw(uint32_t key, uint32_t key2, uint32_t site) {
+    return hash32(hash32(site ^ key) ^ key2);
+}
+
+// Second 24-bit uniform, used when the neighbour count is not a power of two
+// and the pick has consumed the whole first draw.
+GHD uint32_t second_uniform(uint32_t h) { return hash32(h ^ 0x68E31DA4U) >> 8; }
+
+// Colour order of one step: a Fisher-Yates shuffle driven by a hash chain.
+inline void colour_order(uint64_t seed, uint32_t replica, uint64_t step, int colours,
+                         int *order) {
+    for (int c = 0; c < colours; ++c) order[c] = c;
+    uint32_t h = pass_key(seed, replica, step, 0xC0105EEDU);
+    for (int i = colours - 1; i > 0; --i) {
+        h = hash32(h + (uint32_t)i);
+        int j = (int)(((uint64_t)h * (uint64_t)(i + 1)) >> 32);
+        int t = order[i];
+        order[i] = order[j];
+        order[j] = t;
+    }
+}
+
+GHD float bits_float(uint32_t u) { return __builtin_bit_cast(float, u); }
+GHD uint32_t float_bits(float f) { return __builtin_bit_cast(uint32_t, f); }
+
+// 2^x in f32: x = i + f with i the nearest integer and |f| <= 1/2, then a
+// degree-7 Taylor polynomial of e^(f ln 2) in Horner form, scaled by 2^i
+// through the exponent field. Zero below -125, so no result is subnormal.
+GHD float exp2_f32(float x) {
+    if (!(x >= -125.0f)) return 0.0f;
+    if (x > 127.0f) x = 127.0f;
+    int i = (int)x;
+    if ((float)i > x) i -= 1;
+    float f = x - (float)i;
+    if (f > 0.5f) {
+        f = f - 1.0f;
+        i += 1;
+    }
+    float p = 1.5252734e-5f;
+    p = p * f + 1.5403530e-4f;
+    p = p * f + 1.3333558e-3f;
+    p = p * f + 9.6181291e-3f;
+    p = p * f + 5.5504109e-2f;
+    p = p * f + 2.4022651e-1f;
+    p = p * f + 6.9314718e-1f;
+    p = p * f + 1.0f;
+    if (i > 127) i = 127;
+    return p * bits_float((uint32_t)(i + 127) << 23);
+}
+
+// log2(x) in f32 for normal x > 0: x = m 2^e with m in [sqrt(1/2), sqrt(2)),
+// then log2 m = (2 / ln 2) atanh(s), s = (m - 1)/(m + 1), by its odd series.
+GHD float log2_f32(float x) {
+    if (!(x >= 1.17549435e-38f)) return -__builtin_huge_valf();
+    uint32_t b = float_bits(x);
+    int e = (int)((b >> 23) & 0xff) - 127;
+    float m = bits_float((b & 0x7fffffU) | 0x3f800000U);
+    if (m > 1.41421356f) {
+        m = m * 0.5f;
+        e += 1;
+    }
+    float s = (m - 1.0f) / (m + 1.0f);
+    float s2 = s * s;
+    float p = 0.11111111f;
+    p = p * s2 + 0.14285714f;
+    p = p * s2 + 0.2f;
+    p = p * s2 + 0.33333333f;
+    p = p * s2 + 1.0f;
+    return (float)e + (s * p) * 2.8853900f;
+}
+
+// Minimum-image offset of an integer coordinate difference, the integer form
+// of Moments::unwrap's fold (glazier-core/src/moments.rs:38-48).
+GHD int fold_int(int d, int span) {
+    if (span <= 1) return 0;
+    if (2 * d > span) return d - span;
+    if (-2 * d > span) return d + span;
+    return d;
+}
+
+// Label words on the device carry the cell type in their top bits, so the
+// sweep reads a neighbour's type from the lattice without a gather.
+constexpr int kTypeShift = 28;
+constexpr uint32_t kIdMask = (1u << kTypeShift) - 1u;
+GHD uint32_t label_id(uint32_t word) { return word & kIdMask; }
+GHD uint32_t label_type(uint32_t word) { return word >> kTypeShift; }
+
+// Volume and surface of one label share an int64: volume * 2^32 + surface.
+// Deltas add exactly in two's complement, so one 64-bit atomic carries both.
+GHD int64_t pack_vs(int64_t vol, int64_t surf) { return vol * 4294967296LL + surf; }
+GHD int32_t unpack_surf(int64_t p) { return (int32_t)(uint32_t)(uint64_t)p; }
+GHD int64_t unpack_vol(int64_t p) { return (p - (int64_t)unpack_surf(p)) / 4294967296LL; }
+
+}  // namespace ghip

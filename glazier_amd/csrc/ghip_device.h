@@ -1,0 +1,138 @@
+// ghip_device.h - device-side data layout of the engine: the model tables a
+// kernel stages into LDS, and the argument blocks of every kernel family.
+#pragma once
+
+#include "ghip_common.h"
+
+namespace ghip {
+
+constexpr int kMaxTypes = 16;
+constexpr int kMaxSpecies = 4;
+constexpr int kMaxThr = 1024;
+constexpr int kAcceptSlots = 64;
+
+// Model parameters as the kernels read them, in the device type index: type
+// 0 is the medium, whose contact row is the reference's cell_type[0] row.
+struct DeviceTables {
+    float J[kMaxTypes][kMaxTypes];
+    float lam_v[kMaxTypes];
+    float tgt_v[kMaxTypes];
+    float lam_s[kMaxTypes];
+    float tgt_s[kMaxTypes];
+    float chi[kMaxTypes][kMaxSpecies];
+    float max_act[kMaxTypes];
+    float coef_act[kMaxTypes];
+    float ext[kMaxTypes][3];
+    float sec[kMaxTypes][kMaxSpecies];
+    float up[kMaxTypes][kMaxSpecies];
+    double lam_len[kMaxTypes];
+    double tgt_len[kMaxTypes];
+    double div_vol[kMaxTypes];
+    double death[kMaxTypes];
+    uint32_t connected[kMaxTypes];
+    uint32_t piece2d[8];      // bit m: the planar ring mask m is one piece
+    uint32_t adj3d[26];       // ring positions adjacent to each position
+    uint32_t thr[kMaxThr];
+    uint32_t n_thr;
+    float accept_c;
+    uint32_t pad[2];
+};
+
+// Feature bits of a sweep launch.
+enum SweepFlags : uint32_t {
+    kIntegral = 1u << 0,
+    kConnected = 1u << 1,
+    kLength = 1u << 2,
+    kChemotaxis = 1u << 3,
+    kMotility = 1u << 4,
+    kDrift = 1u << 5,
+    kMoments = 1u << 6,  // per-label moment sums are kept (length or division)
+};
+
+// One colour pass over one lattice.
+struct SweepArgs {
+    uint32_t *labels;            // packed label words, n sites
+    const uint32_t *labels_prev; // pass-start copy, read at distance two (motility)
+    float *act;                  // activity, written on accept
+    const float *act_prev;       // pass-start copy
+    const float *fields[kMaxSpecies]; // current buffer of each species
+    const int64_t *a_cur;        // per-label base snapshot
+    const int64_t *e_prev;       // delta of the previous pass
+    int64_t *f_acc;              // delta this pass accumulates
+    int64_t *a_next;             // base of the next pass: a_cur + e_prev
+    int64_t *g_zero;             // delta buffer of the pass after next, cleared
+    const int32_t *anchor;       // 3 per label
+    const DeviceTables *tables;
+    const uint32_t *n_labels;    // device label count
+    unsigned long long *accepted; // kAcceptSlots counters
+    int w, h, d;
+    int cx, cy, cz;
+    uint32_t key, key2;
+    int tx_log2, ty_log2, tz_log2; // tile size in targets
+    int halo;
+    int n_off;
+    uint32_t n_types;
+    uint32_t n_thr;
+    int cell_words;
+    int n_species;
+    uint32_t flags;
+    uint32_t fold_per_block;
+    int xcd_chunk;               // tiles per XCD in the remap, 0 for none
+    int tiles_x, tiles_y, tiles_z;
+    int off[26][3];
+};
+
+// One launch of the field step: `steps` substeps of one species.
+struct FieldArgs {
+    const float *in;        // one species, n sites
+    float *out;
+    const uint32_t *labels; // packed words; type in the top bits
+    const DeviceTables *tables;
+    int w, h, d;
+    int ix_log2, iy_log2, iz; // interior tile: 2^ix x 2^iy x iz sites
+    int steps;                // substeps this launch takes
+    int exchange;             // fuse secretion/uptake after the last one
+    int species;
+    float dcoef, kcoef;
+    int tiles_x, tiles_y;
+};
+
+// Per-label bookkeeping launches (recounts, anchors, division/death).
+struct CellArgs {
+    uint32_t *labels;      // packed words
+    int64_t *state;        // consolidated per-label state
+    int32_t *anchor;       // 3 per label
+    uint32_t *n_labels;    // device label count
+    uint32_t *ltype;       // per label: its packed word (type bits | id)
+    uint32_t *scratch_u32; // first sites / moved counts / remap, capacity entries
+    uint32_t *daughter;    // parent -> daughter id, capacity entries
+    uint8_t *plan;         // capacity entries
+    double *geo;           // 6 per label
+    uint32_t *flags;       // [0] event, [1] recentre, [2] overflow, [3] compacted count
+    const DeviceTables *tables;
+    int w, h, d;
+    int cw;
+    int n_off;
+    int off[26][3];
+    uint32_t capacity;
+    uint32_t key, key2;    // death draw key of this step
+};
+
+// Launchers, one per kernel family (sweep.hip, fields.hip, cells.hip).
+hipError_t launch_sweep(const SweepArgs &a, bool d3, bool full, int blocks, int threads,
+                        hipStream_t stream);
+size_t sweep_lds_bytes(const SweepArgs &a, bool d3);
+hipError_t launch_field(const FieldArgs &a, bool d3, int blocks, hipStream_t stream);
+size_t field_lds_bytes(const FieldArgs &a, bool d3);
+hipError_t launch_activity_decay(float *act, size_t n, hipStream_t stream);
+hipError_t launch_fill_i64(int64_t *p, size_t n, hipStream_t s);
+hipError_t launch_fill_u32(uint32_t *p, size_t n, uint32_t v, hipStream_t s);
+hipError_t launch_consolidate(int64_t *a, int64_t *e, const uint32_t *n_labels, int cw, size_t cap,
+                              hipStream_t s);
+hipError_t launch_recount(const CellArgs &a, size_t n_sites, hipStream_t s);
+hipError_t launch_initial_anchors(const CellArgs &a, size_t n_sites, hipStream_t s);
+hipError_t launch_recentre_check(const CellArgs &a, size_t n_sites, hipStream_t s);
+hipError_t launch_population(const CellArgs &a, size_t n_sites, uint32_t *type_copy,
+                             int32_t *anchor_copy, hipStream_t s);
+
+}  // namespace ghip

@@ -1,0 +1,210 @@
+"""Python side of the HIP engine: a ctypes binding of include/glazier_hip.h.
+
+`HipSimulation` is the counterpart of GpuSimulation
+(glazier-cuda/src/lib.rs:749-1569): built from a model and a starting lattice,
+stepped in whole Monte Carlo steps, read back into numpy arrays. The compiled
+library `libglazier_hip.so` sits beside this file; without it, importing the
+engine raises rather than falling back to anything else.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+from pathlib import Path
+from typing import Optional, Sequence
+
+import numpy as np
+
+from .model import CModel, Model
+
+LIB_PATH = Path(__file__).resolve().parent / "libglazier_hip.so"
+
+GHIP_OK = 0
+GHIP_ODD_LATTICE = 1
+GHIP_DEVICE = 2
+GHIP_UNSUPPORTED = 3
+GHIP_BAD_ARG = 4
+
+
+class GhipError(RuntimeError):
+    """A failure the engine reported, with its status code."""
+
+    def __init__(self, code: int, message: str):
+        super().__init__(message)
+        self.code = code
+
+
+_lib = None
+
+
+def library() -> ctypes.CDLL:
+    """The compiled engine. Raises when it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise ImportError(
+            f"{LIB_PATH} is not built; run __graft_entry__.build() (hipcc for gfx950)")
+    lib = ctypes.CDLL(str(LIB_PATH))
+    P = ctypes.POINTER
+    sim = ctypes.c_void_p
+    lib.ghip_create.argtypes = [P(CModel), P(ctypes.c_uint32), ctypes.c_size_t, P(ctypes.c_uint8),
+                                ctypes.c_size_t, P(ctypes.c_double), P(ctypes.c_double),
+                                P(ctypes.c_void_p)]
+    lib.ghip_destroy.argtypes = [sim]
+    lib.ghip_step.argtypes = [sim, ctypes.c_uint64]
+    lib.ghip_mcs.argtypes = [sim]
+    lib.ghip_mcs.restype = ctypes.c_uint64
+    lib.ghip_n_labels.argtypes = [sim, P(ctypes.c_size_t)]
+    lib.ghip_labels.argtypes = [sim, P(ctypes.c_uint32), ctypes.c_size_t]
+    lib.ghip_volumes.argtypes = [sim, P(ctypes.c_uint32), ctypes.c_size_t]
+    lib.ghip_surfaces.argtypes = [sim, P(ctypes.c_int64), ctypes.c_size_t]
+    lib.ghip_cell_types.argtypes = [sim, P(ctypes.c_uint8), ctypes.c_size_t]
+    lib.ghip_field.argtypes = [sim, ctypes.c_uint32, P(ctypes.c_double), ctypes.c_size_t]
+    lib.ghip_field_f32.argtypes = [sim, ctypes.c_uint32, P(ctypes.c_float), ctypes.c_size_t]
+    lib.ghip_set_field.argtypes = [sim, ctypes.c_uint32, P(ctypes.c_double), ctypes.c_size_t]
+    lib.ghip_activity_f32.argtypes = [sim, P(ctypes.c_float), ctypes.c_size_t]
+    lib.ghip_counters.argtypes = [sim, P(ctypes.c_uint64), P(ctypes.c_uint64)]
+    lib.ghip_cell_state.argtypes = [sim, P(ctypes.c_int64), ctypes.c_size_t, P(ctypes.c_uint32)]
+    lib.ghip_set_timing.argtypes = [sim, ctypes.c_int]
+    lib.ghip_kernel_time.argtypes = [sim, ctypes.c_char_p, P(ctypes.c_double), P(ctypes.c_uint64)]
+    lib.ghip_last_error.restype = ctypes.c_char_p
+    _lib = lib
+    return lib
+
+
+def _check(status: int) -> None:
+    if status != GHIP_OK:
+        msg = library().ghip_last_error().decode()
+        raise GhipError(status, msg)
+
+
+def _ptr(array: np.ndarray, ctype):
+    return array.ctypes.data_as(ctypes.POINTER(ctype))
+
+
+class HipSimulation:
+    """A run on one MI355X, seeded from a model and a lattice.
+
+    `labels` is the flat label field (x fastest); `cell_types` one type per
+    label, the medium's included (the reference keeps 1 there,
+    glazier-core/src/cpu.rs:203); `fields` species-major f64 values or None
+    for each species' initial value; `activity` per-site memory or None.
+    """
+
+    def __init__(self, model: Model, labels: np.ndarray, cell_types: np.ndarray,
+                 fields: Optional[np.ndarray] = None, activity: Optional[np.ndarray] = None):
+        lib = library()
+        self.model = model
+        self._c = model.to_c()
+        lab = np.ascontiguousarray(labels, dtype=np.uint32).reshape(-1)
+        ct = np.ascontiguousarray(cell_types, dtype=np.uint8).reshape(-1)
+        self.sites = lab.size
+        fptr = None
+        if fields is not None:
+            f = np.ascontiguousarray(fields, dtype=np.float64).reshape(-1)
+            fptr = _ptr(f, ctypes.c_double)
+        aptr = None
+        if activity is not None:
+            a = np.ascontiguousarray(activity, dtype=np.float64).reshape(-1)
+            aptr = _ptr(a, ctypes.c_double)
+        handle = ctypes.c_void_p()
+        _check(lib.ghip_create(ctypes.byref(self._c), _ptr(lab, ctypes.c_uint32), lab.size,
+                               _ptr(ct, ctypes.c_uint8), ct.size, fptr, aptr, ctypes.byref(handle)))
+        self._h = handle
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) and self._h.value:
+            library().ghip_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- stepping ---------------------------------------------------------
+    def step(self, steps: int = 1) -> None:
+        _check(library().ghip_step(self._h, int(steps)))
+
+    @property
+    def mcs(self) -> int:
+        return int(library().ghip_mcs(self._h))
+
+    # -- read-backs -------------------------------------------------------
+    def n_labels(self) -> int:
+        n = ctypes.c_size_t()
+        _check(library().ghip_n_labels(self._h, ctypes.byref(n)))
+        return int(n.value)
+
+    def labels(self) -> np.ndarray:
+        out = np.empty(self.sites, dtype=np.uint32)
+        _check(library().ghip_labels(self._h, _ptr(out, ctypes.c_uint32), out.size))
+        return out
+
+    def volumes(self) -> np.ndarray:
+        out = np.empty(self.n_labels(), dtype=np.uint32)
+        _check(library().ghip_volumes(self._h, _ptr(out, ctypes.c_uint32), out.size))
+        return out
+
+    def surfaces(self) -> np.ndarray:
+        out = np.empty(self.n_labels(), dtype=np.int64)
+        _check(library().ghip_surfaces(self._h, _ptr(out, ctypes.c_int64), out.size))
+        return out
+
+    def cell_types(self) -> np.ndarray:
+        out = np.empty(self.n_labels(), dtype=np.uint8)
+        _check(library().ghip_cell_types(self._h, _ptr(out, ctypes.c_uint8), out.size))
+        return out
+
+    def field(self, species: int) -> np.ndarray:
+        out = np.empty(self.sites, dtype=np.float64)
+        _check(library().ghip_field(self._h, species, _ptr(out, ctypes.c_double), out.size))
+        return out
+
+    def field_f32(self, species: int) -> np.ndarray:
+        out = np.empty(self.sites, dtype=np.float32)
+        _check(library().ghip_field_f32(self._h, species, _ptr(out, ctypes.c_float), out.size))
+        return out
+
+    def set_field(self, species: int, values: np.ndarray) -> None:
+        v = np.ascontiguousarray(values, dtype=np.float64).reshape(-1)
+        _check(library().ghip_set_field(self._h, species, _ptr(v, ctypes.c_double), v.size))
+
+    def activity_f32(self) -> np.ndarray:
+        out = np.empty(self.sites, dtype=np.float32)
+        _check(library().ghip_activity_f32(self._h, _ptr(out, ctypes.c_float), out.size))
+        return out
+
+    def counters(self):
+        att, acc = ctypes.c_uint64(), ctypes.c_uint64()
+        _check(library().ghip_counters(self._h, ctypes.byref(att), ctypes.byref(acc)))
+        return int(att.value), int(acc.value)
+
+    def cell_state(self) -> np.ndarray:
+        """Per label: volume, surface, then the nine integer moment sums when
+        the engine keeps them; shape (n_labels, words)."""
+        words = ctypes.c_uint32()
+        lib = library()
+        _check(lib.ghip_cell_state(self._h, None, 0, ctypes.byref(words)))
+        n = self.n_labels()
+        out = np.empty(n * words.value, dtype=np.int64)
+        _check(lib.ghip_cell_state(self._h, _ptr(out, ctypes.c_int64), out.size, ctypes.byref(words)))
+        return out.reshape(n, words.value)
+
+    # -- timing -----------------------------------------------------------
+    def set_timing(self, on: bool) -> None:
+        _check(library().ghip_set_timing(self._h, 1 if on else 0))
+
+    def kernel_time(self, family: str):
+        ms, n = ctypes.c_double(), ctypes.c_uint64()
+        _check(library().ghip_kernel_time(self._h, family.encode(), ctypes.byref(ms), ctypes.byref(n)))
+        return float(ms.value), int(n.value)

@@ -1,0 +1,59 @@
+"""The synthetic workloads the engine is measured and checked on.
+
+Each builds a Model, a starting lattice and a type per label:
+
+  plain_monolayer(side, seed, surface)  a periodic sheet of side x side sites
+      tiled by 8 x 8 cells, one cell type, contact J = 10 on every unlike
+      Moore bond, lambda_V = 2, V_t = 64, T = 10; with `surface`, lambda_S = 1
+      and S_t = 32. 128^2 without the surface term is the CPU-runnable case;
+      1024^2 to 4096^2 with it is the throughput case.
+  infection_sheet(side, seed)  two cell types (epithelial 95 %, infected 5 %
+      placed by seed), two f32 fields (virus D 0.1 decay 0.01, interferon D
+      0.05 decay 0.02) on 4 substeps, secretion 1.0 / 0.5 by infected cells,
+      uptake 0.05 by epithelial cells, chemotaxis 5 toward interferon, length
+      lambda_L 1 L_t 10, infected death 0.002 per step, division at 128.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+from .model import CellType, Model, Species, default_cell_types, tiled_labels
+
+
+def plain_monolayer(side: int = 128, seed: int = 1, surface: bool = False):
+    cell = CellType(target_volume=64.0, lambda_volume=2.0)
+    if surface:
+        cell.target_surface = 32.0
+        cell.lambda_surface = 1.0
+    model = Model(width=side, height=side, depth=1, contact=[0.0, 10.0, 10.0, 10.0],
+                  types=[CellType(), cell], temperature=10.0, neighbour_order=2, seed=seed)
+    n = side // 8
+    labels = tiled_labels(model, 8, n, n, 1)
+    types = default_cell_types(n * n + 1)
+    return model, labels, types
+
+
+def infection_sheet(side: int = 512, seed: int = 7, infected_fraction: float = 0.05):
+    epi = CellType(target_volume=64.0, lambda_volume=2.0, target_length=10.0, lambda_length=1.0)
+    inf = CellType(target_volume=64.0, lambda_volume=2.0, target_length=10.0, lambda_length=1.0,
+                   death_rate=0.002, division_volume=128.0)
+    model = Model(
+        width=side, height=side, depth=1,
+        contact=[0.0, 10.0, 10.0, 10.0, 10.0, 10.0, 10.0, 10.0, 10.0],
+        types=[CellType(), epi, inf], temperature=10.0, neighbour_order=2, seed=seed,
+        species=[Species("virus", 0.1, 0.01, 0.0, substeps=4),
+                 Species("interferon", 0.05, 0.02, 0.0, substeps=4)],
+        secretion=[[0.0, 0.0], [0.0, 0.0], [1.0, 0.5]],
+        uptake=[[0.0, 0.0], [0.05, 0.05], [0.0, 0.0]],
+        chemotaxis=[[0.0, 0.0], [0.0, 5.0], [0.0, 5.0]],
+    )
+    n = side // 8
+    cells = n * n
+    labels = tiled_labels(model, 8, n, n, 1)
+    rng = np.random.default_rng(seed)
+    infected = rng.choice(cells, size=int(round(infected_fraction * cells)), replace=False)
+    kinds = np.ones(cells, dtype=np.uint8)
+    kinds[infected] = 2
+    types = default_cell_types(cells + 1, kinds)
+    return model, labels, types

@@ -1,0 +1,172 @@
+/*
+ * glazier_hip.h - C ABI of the MI355X (gfx950) Cellular Potts engine.
+ *
+ * The engine is a drop-in for the device path of glazier: the checkerboard
+ * Metropolis sweep, the chemical-field stencil, the per-cell moments and
+ * division/death. Every entry point below replaces one member of
+ * `GpuSimulation` in glazier-cuda/src/lib.rs, cited beside it, and keeps its
+ * argument meaning and its error behaviour.
+ *
+ * Conventions:
+ *   - plain C: pointers and sizes, no exceptions cross the boundary;
+ *   - every call returns a ghip_status; the message of the last failure on
+ *     the calling thread is ghip_last_error();
+ *   - host buffers stay owned by the caller; the handle owns device memory;
+ *   - one handle per host thread at a time; distinct handles are independent.
+ *
+ * Lattice order is x fastest, then y, then z (glazier-core/src/lattice.rs:122).
+ */
+#ifndef GLAZIER_HIP_H
+#define GLAZIER_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GHIP_MAX_TYPES 16
+#define GHIP_MAX_SPECIES 4
+
+/* Status codes. GHIP_ODD_LATTICE carries the reference's "even sides"
+ * wording (glazier-cuda/src/lib.rs:733-736). */
+typedef enum ghip_status {
+    GHIP_OK = 0,
+    GHIP_ODD_LATTICE = 1,
+    GHIP_DEVICE = 2,      /* a HIP call failed */
+    GHIP_UNSUPPORTED = 3, /* a term the engine refuses rather than drops */
+    GHIP_BAD_ARG = 4      /* a null pointer or a length that does not match */
+} ghip_status;
+
+/* One cell type: a flat mirror of CellType (glazier-core/src/model.rs:9-41).
+ * Type 0 is the medium and takes no constraint. */
+typedef struct ghip_cell_type {
+    double target_volume;
+    double lambda_volume;
+    double target_surface;
+    double lambda_surface;
+    double division_volume;
+    double death_rate;
+    double target_length;
+    double lambda_length;
+    double max_activity;
+    double lambda_activity;
+    double external[3];
+    double lambda_nematic;
+    int32_t connected;
+    int32_t reserved;
+} ghip_cell_type;
+
+/* One diffusing species (glazier-core/src/field.rs:15-25). `substeps` 0 asks
+ * for the stability count ceil(D / (1/(2 dims))) (field.rs:155-160); a
+ * positive value states the count outright. */
+typedef struct ghip_species {
+    double diffusion;
+    double decay;
+    double initial;
+    int32_t substeps;
+    int32_t reserved;
+} ghip_species;
+
+/* The whole model: a flat mirror of Model (glazier-core/src/model.rs:44-76)
+ * with fixed-size tables, so it crosses the ABI as one POD. */
+typedef struct ghip_model {
+    uint32_t width;
+    uint32_t height;
+    uint32_t depth;           /* 1 is a plane */
+    uint32_t neighbour_order; /* 1, 2 or 3 (model.rs:183-188) */
+    uint32_t n_types;         /* including the medium */
+    uint32_t n_species;
+    double temperature;
+    uint64_t seed;
+    uint32_t replica;         /* stream id mixed into every draw */
+    uint32_t has_nematic;     /* a nematic field and a coupling are present */
+    double contact[GHIP_MAX_TYPES * GHIP_MAX_TYPES]; /* row-major, n_types^2 used */
+    ghip_cell_type types[GHIP_MAX_TYPES];
+    ghip_species species[GHIP_MAX_SPECIES];
+    double secretion[GHIP_MAX_TYPES][GHIP_MAX_SPECIES];
+    double uptake[GHIP_MAX_TYPES][GHIP_MAX_SPECIES];
+    double chemotaxis[GHIP_MAX_TYPES][GHIP_MAX_SPECIES];
+} ghip_model;
+
+typedef struct ghip_sim ghip_sim;
+
+/* Replaces GpuSimulation::from_cpu (glazier-cuda/src/lib.rs:814-1049).
+ * `labels` has n_sites = width*height*depth entries; `cell_type` has one
+ * entry per label including the medium at index 0 (the reference keeps
+ * cell_type[0] = 1, glazier-core/src/cpu.rs:203, and the medium then prices
+ * contact as that type); `fields` is n_species*n_sites values, species-major,
+ * or NULL for every species' initial value; `activity` is n_sites values or
+ * NULL for none. Fails with GHIP_ODD_LATTICE on an odd side, and with
+ * GHIP_UNSUPPORTED on a nematic model or a term outside this engine. */
+ghip_status ghip_create(const ghip_model *model, const uint32_t *labels, size_t n_sites,
+                        const uint8_t *cell_type, size_t n_labels, const double *fields,
+                        const double *activity, ghip_sim **out);
+
+/* Frees the handle and its device memory (the Drop of GpuSimulation). */
+ghip_status ghip_destroy(ghip_sim *sim);
+
+/* Replaces GpuSimulation::step (glazier-cuda/src/lib.rs:1055-1144): runs
+ * `steps` Monte Carlo steps and returns once they are done. */
+ghip_status ghip_step(ghip_sim *sim, uint64_t steps);
+
+/* Replaces the public field GpuSimulation::mcs (lib.rs:805). */
+uint64_t ghip_mcs(const ghip_sim *sim);
+
+/* Labels, medium included, that the per-label read-backs below cover. It
+ * grows when cells divide and shrinks when labels are compacted. */
+ghip_status ghip_n_labels(const ghip_sim *sim, size_t *out);
+
+/* Replaces GpuSimulation::labels (lib.rs:1227-1229): n must be n_sites. */
+ghip_status ghip_labels(const ghip_sim *sim, uint32_t *out, size_t n);
+
+/* Replaces GpuSimulation::volumes (lib.rs:1258-1269): n must be n_labels. */
+ghip_status ghip_volumes(const ghip_sim *sim, uint32_t *out, size_t n);
+
+/* Replaces GpuSimulation::surfaces (lib.rs:1245-1255): n must be n_labels. */
+ghip_status ghip_surfaces(const ghip_sim *sim, int64_t *out, size_t n);
+
+/* Cell type per label, medium included: n must be n_labels. */
+ghip_status ghip_cell_types(const ghip_sim *sim, uint8_t *out, size_t n);
+
+/* Replaces GpuSimulation::field (lib.rs:1235-1242): one species as f64. */
+ghip_status ghip_field(const ghip_sim *sim, uint32_t species, double *out, size_t n);
+
+/* The same species as the device holds it, f32, for bit-level comparison. */
+ghip_status ghip_field_f32(const ghip_sim *sim, uint32_t species, float *out, size_t n);
+
+/* Replaces the fields the reference uploads once (lib.rs:951-966): sets one
+ * species from f64 values, n = n_sites. */
+ghip_status ghip_set_field(ghip_sim *sim, uint32_t species, const double *values, size_t n);
+
+/* Persistent-motility memory per site, f32 as held (motility.rs:18-22). */
+ghip_status ghip_activity_f32(const ghip_sim *sim, float *out, size_t n);
+
+/* Copy attempts made and accepted since creation (cpu.rs:99-102). */
+ghip_status ghip_counters(const ghip_sim *sim, uint64_t *attempted, uint64_t *accepted);
+
+/* Per-label bookkeeping as the engine keeps it, for bit-level comparison:
+ * `words` int64 per label in the order volume, surface, then the integer
+ * moment sums n-free (sx, sy, sz, sxx, syy, szz, sxy, sxz, syz) about each
+ * label's anchor when the length term is on. Returns the word count per
+ * label in *words_per_label; `out` may be NULL to ask for it. */
+ghip_status ghip_cell_state(const ghip_sim *sim, int64_t *out, size_t n,
+                            uint32_t *words_per_label);
+
+/* Kernel timing over the stream the engine launches on: when on, every sweep
+ * pass, field step and moment rebuild is bracketed by HIP events.
+ * ghip_kernel_time returns the summed milliseconds and the launch count of
+ * one kernel family: "sweep", "fields", "moments" or "population". */
+ghip_status ghip_set_timing(ghip_sim *sim, int on);
+ghip_status ghip_kernel_time(const ghip_sim *sim, const char *family, double *total_ms,
+                             uint64_t *launches);
+
+/* Message of the last failure on this thread; empty when none. */
+const char *ghip_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GLAZIER_HIP_H */
