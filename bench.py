@@ -69,6 +69,9 @@ def main() -> int:
     ap.add_argument("--side", type=int, default=4096)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only copy a MIB-mebibyte buffer 5 times with torch, so a counter "
+                         "run can scale FETCH_SIZE/WRITE_SIZE by a copy of known size")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported by this build",
@@ -76,6 +79,16 @@ def main() -> int:
         return 2
 
     import torch  # device memory plumbing only: the engine allocates its own
+
+    if args.copy_probe:
+        n = args.copy_probe * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": n * 4, "copies": 5}))
+        return 0
     from glazier_amd import HipSimulation
     from glazier_amd.workloads import plain_monolayer
 

@@ -15,7 +15,7 @@ from pathlib import Path
 HERE = Path(__file__).resolve().parent
 CSRC = HERE / "csrc"
 OUT = HERE / "libglazier_hip.so"
-SOURCES = ["sweep.hip", "fields.hip", "cells.hip", "engine.cpp"]
+SOURCES = ["sweep.hip", "sweep_plain.hip", "fields.hip", "cells.hip", "engine.cpp"]
 FLAGS = [
     "--offload-arch=gfx950",
     "-O3",

@@ -103,6 +103,7 @@ struct ghip_sim {
 
     int tx_log2 = 6, ty_log2 = 3, tz_log2 = 0, threads = 512;
     int tiles_x = 1, tiles_y = 1, tiles_z = 1, blocks = 1;
+    bool plain_fast = false;  // the 128 x 16 plain-monolayer pass applies
     int fx_log2 = 6, fy_log2 = 5, fz = 1, f_tiles_x = 1, f_tiles_y = 1, f_blocks = 1, f_kmax = 4;
 
     bool timing = false;
@@ -343,6 +344,16 @@ void choose_tiles(ghip_sim *s) {
     }
     s->threads = ((s->threads + 63) / 64) * 64;
     s->blocks = s->tiles_x * s->tiles_y * s->tiles_z;
+    s->plain_fast = !s->d3 && s->n_off == 8 && s->m.n_types == 2 &&
+                    (s->sweep_flags & kIntegral) && !s->full && s->W % 128 == 0 && s->H % 16 == 0 &&
+                    plain2d_supported(s->host_tables.n_thr);
+    if (s->plain_fast) {
+        s->tiles_x = s->W / 128;
+        s->tiles_y = s->H / 16;
+        s->tiles_z = 1;
+        s->blocks = s->tiles_x * s->tiles_y;
+        s->threads = 256;
+    }
     // fields: 64 x 32 interior in a plane, 16 x 8 x 8 in a volume
     if (!s->d3) {
         s->fx_log2 = ilog2(std::min(64, pow2_floor(s->W)));
@@ -413,7 +424,20 @@ ghip_status run_sweep_pass(ghip_sim *s, int colour) {
     std::memcpy(a.off, s->off, sizeof s->off);
     TimedLaunch t{};
     HIP_TRY(s->begin_timed(0, &t));
-    HIP_TRY(launch_sweep(a, s->d3, s->full, s->blocks, s->threads, s->stream));
+    if (s->plain_fast) {
+        PlainParams pp{};
+        for (int x = 0; x < 2; ++x)
+            for (int y = 0; y < 2; ++y) pp.j[x][y] = s->host_tables.J[x][y];
+        pp.lam_v = s->host_tables.lam_v[1];
+        pp.tgt_v = s->host_tables.tgt_v[1];
+        pp.lam_s = s->host_tables.lam_s[1];
+        pp.tgt_s = s->host_tables.tgt_s[1];
+        pp.thr = s->tables->thr;
+        pp.n_thr = s->host_tables.n_thr;
+        HIP_TRY(launch_sweep_plain2d(a, pp, s->blocks, s->stream));
+    } else {
+        HIP_TRY(launch_sweep(a, s->d3, s->full, s->blocks, s->threads, s->stream));
+    }
     HIP_TRY(s->end_timed(t));
     s->pass += 1;
     if (mot) {

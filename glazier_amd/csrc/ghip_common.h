@@ -132,6 +132,6 @@ GHD uint32_t label_type(uint32_t word) { return word >> kTypeShift; }
 // Deltas add exactly in two's complement, so one 64-bit atomic carries both.
 GHD int64_t pack_vs(int64_t vol, int64_t surf) { return vol * 4294967296LL + surf; }
 GHD int32_t unpack_surf(int64_t p) { return (int32_t)(uint32_t)(uint64_t)p; }
-GHD int64_t unpack_vol(int64_t p) { return (p - (int64_t)unpack_surf(p)) / 4294967296LL; }
+GHD int64_t unpack_vol(int64_t p) { return (p - (int64_t)unpack_surf(p)) >> 32; }
 
 }  // namespace ghip

@@ -82,6 +82,14 @@ struct SweepArgs {
     int off[26][3];
 };
 
+// Scalars of the plain-monolayer pass: the medium and one cell type.
+struct PlainParams {
+    float j[2][2];          // device contact table
+    float lam_v, tgt_v, lam_s, tgt_s;
+    const uint32_t *thr;    // integral acceptance table
+    uint32_t n_thr;
+};
+
 // One launch of the field step: `steps` substeps of one species.
 struct FieldArgs {
     const float *in;        // one species, n sites
@@ -122,6 +130,9 @@ struct CellArgs {
 hipError_t launch_sweep(const SweepArgs &a, bool d3, bool full, int blocks, int threads,
                         hipStream_t stream);
 size_t sweep_lds_bytes(const SweepArgs &a, bool d3);
+hipError_t launch_sweep_plain2d(const SweepArgs &a, const PlainParams &p, int blocks,
+                                hipStream_t stream);
+bool plain2d_supported(uint32_t n_thr);
 hipError_t launch_field(const FieldArgs &a, bool d3, int blocks, hipStream_t stream);
 size_t field_lds_bytes(const FieldArgs &a, bool d3);
 hipError_t launch_activity_decay(float *act, size_t n, hipStream_t stream);

@@ -1,0 +1,215 @@
+"""The CPU emulation of the device schedule (oracle/emu.c), pinned against
+the serial oracle and against closed forms.
+
+The emulation is the bit-exact reference of the HIP kernels, so its own
+pricing, bookkeeping, counter hash and field arithmetic are checked here:
+energy changes against the serial engine's full recomputation, counters
+against recounts, the hash's uniformity, f32 fields against the f64 serial
+solver (relative L2 1e-5), and the serial engine's distribution over 32 seeds
+(gpu_matches_cpu.rs:48-92).
+"""
+
+import ctypes
+
+import numpy as np
+import pytest
+
+import oracle
+from glazier_amd.model import CellType, Model, Species, default_cell_types, tiled_labels
+from tests.cases import case, ref_model, tiled
+
+
+def lengths_from_state(state, three_d=False):
+    """4 sqrt(lambda_max(cov)) from integer sums, in f64 (numpy eigvalsh)."""
+    out = np.zeros(len(state))
+    for l, row in enumerate(state):
+        n = row[0]
+        if n < 2 or len(row) < 11:
+            continue
+        sx, sy, sz, sxx, syy, szz, sxy, sxz, syz = [float(v) for v in row[2:11]]
+        c = np.array([[n * sxx - sx * sx, n * sxy - sx * sy, n * sxz - sx * sz],
+                      [n * sxy - sx * sy, n * syy - sy * sy, n * syz - sy * sz],
+                      [n * sxz - sx * sz, n * syz - sy * sz, n * szz - sz * sz]]) / (n * n)
+        out[l] = 4.0 * np.sqrt(max(np.linalg.eigvalsh(c)[-1], 0.0))
+    return out
+
+
+def lengths_from_sites(labels, m):
+    """Major axis of every label from its sites, unwrapped about its first
+    site by minimum image, with numpy's symmetric eigensolver."""
+    d, h, w = m.depth, m.height, m.width
+    idx = np.arange(labels.size)
+    coords = np.stack([idx % w, (idx // w) % h, idx // (w * h)], axis=1).astype(np.float64)
+    span = np.array([w, h, d], dtype=np.float64)
+    out = np.zeros(int(labels.max()) + 1)
+    for l in np.unique(labels):
+        if l == 0:
+            continue
+        c = coords[labels == l]
+        if len(c) < 2:
+            continue
+        delta = c - c[0]
+        for ax in range(3):
+            if span[ax] > 1:
+                delta[:, ax] -= span[ax] * np.round(delta[:, ax] / span[ax])
+        cov = np.cov(delta.T, bias=True)
+        out[l] = 4.0 * np.sqrt(max(np.linalg.eigvalsh(cov)[-1], 0.0))
+    return out
+
+
+@pytest.mark.parametrize("name", ["ref64", "surface256", "length", "division", "infection128",
+                                  "volume3d_o2", "motility", "connected"])
+def test_counters_equal_recounts(oracle_lib, name):
+    m, labels, types, fields, activity, steps = case(name)
+    emu = oracle.EmuSimulation(m, labels, types, fields, activity)
+    emu.step(steps)
+    lab = emu.labels()
+    ser = oracle.SerialSimulation(m, lab, emu.cell_types(), fields)
+    assert np.array_equal(emu.volumes(), ser.recounted_volumes())
+    assert np.array_equal(emu.surfaces(), ser.recounted_surfaces())
+    state = emu.cell_state()
+    if state.shape[1] == 11:
+        live = emu.volumes() > 1
+        live[0] = False
+        want = lengths_from_sites(lab, m)
+        got = lengths_from_state(state)
+        assert np.allclose(got[live], want[live], rtol=1e-9, atol=1e-9)
+
+
+def test_integer_energy_changes_match_the_serial_engine_exactly(oracle_lib):
+    # emulation pricing (f32, exact for integer energies) against the serial
+    # engine's recomputation of the whole energy (cpu_reference.rs:32-74)
+    m, labels, types, _, _, _ = case("surface256")
+    emu = oracle.EmuSimulation(m, labels, types)
+    emu.step(5)
+    lab = emu.labels()
+    ser = oracle.SerialSimulation(m, lab, emu.cell_types())
+    rng = np.random.default_rng(1)
+    w = m.width
+    checked = 0
+    for _ in range(400):
+        t = int(rng.integers(lab.size))
+        x, y = t % w, t // w
+        dx, dy = rng.integers(-1, 2, size=2)
+        s = ((y + dy) % m.height) * w + (x + dx) % w
+        if lab[t] == lab[s] or s == t:
+            continue
+        got = emu.delta_energy(t, s)
+        before = ser.energy()
+        after = ser.clone()
+        after.apply_copy(t, int(lab[s]))
+        want = after.energy() - before
+        after.close()
+        assert got == want, (t, s, got, want)
+        checked += 1
+    assert checked > 50
+
+
+def test_length_energy_changes_match_exact_eigenvalues(oracle_lib):
+    # the length term from the emulation's integer moments against the same
+    # term computed from scratch: contact/volume/surface from the serial
+    # engine with the length term off, plus lambda_L((L'-L_t)^2 - (L-L_t)^2)
+    # with L from numpy's eigensolver on the cells' sites before and after
+    import copy
+
+    m, labels, types, _, _, _ = case("length")
+    emu = oracle.EmuSimulation(m, labels, types)
+    emu.step(3)
+    lab = emu.labels()
+    m0 = copy.deepcopy(m)
+    m0.types[1].lambda_length = 0.0
+    ser = oracle.SerialSimulation(m0, lab, emu.cell_types())
+    lam, lt = m.types[1].lambda_length, m.types[1].target_length
+    before = lengths_from_sites(lab, m)
+    rng = np.random.default_rng(2)
+    w = m.width
+    checked = 0
+    for _ in range(500):
+        t = int(rng.integers(lab.size))
+        x, y = t % w, t // w
+        dx, dy = rng.integers(-1, 2, size=2)
+        s_ = int(((y + dy) % m.height) * w + (x + dx) % w)
+        old, new = int(lab[t]), int(lab[s_])
+        if old == new:
+            continue
+        after_lab = lab.copy()
+        after_lab[t] = new
+        after = lengths_from_sites(after_lab, m)
+        want = ser.delta_energy(t, new)
+        for c in (old, new):
+            if c != 0:
+                want += lam * ((after[c] - lt) ** 2 - (before[c] - lt) ** 2)
+        got = emu.delta_energy(t, s_)
+        assert abs(got - want) <= 2e-5 * max(1.0, abs(want)), (got, want)
+        checked += 1
+    assert checked > 30
+
+
+def test_the_emulation_is_deterministic(oracle_lib):
+    m, labels, types, *_ = case("infection128")
+    a = oracle.EmuSimulation(m, labels, types)
+    b = oracle.EmuSimulation(m, labels, types)
+    a.step(10)
+    b.step(10)
+    assert np.array_equal(a.labels(), b.labels())
+    assert np.array_equal(a.field_f32(0), b.field_f32(0))
+
+
+def test_counter_hash_is_a_bijection_and_uniform(oracle_lib):
+    xs = np.arange(1 << 16, dtype=np.uint64) * 65537 + 12345
+    hs = {oracle.hash32(int(x) & 0xFFFFFFFF) for x in xs[:20000]}
+    assert len(hs) == 20000
+    lib = oracle.library()
+    key = lib.oracle_pass_key(7, 0, 3, 1)
+    draws = np.array([lib.oracle_site_draw(key, s) for s in range(65536)], dtype=np.uint64)
+    picks = np.bincount((draws & 7).astype(np.int64), minlength=8)
+    chi2 = ((picks - 8192.0) ** 2 / 8192.0).sum()
+    assert chi2 < 30.0  # 7 dof, p ~ 1e-4
+    top = np.bincount((draws >> 24).astype(np.int64), minlength=256)
+    chi2 = ((top - 256.0) ** 2 / 256.0).sum()
+    assert chi2 < 340.0  # 255 dof, p ~ 3e-4
+    # neighbouring sites' draws are uncorrelated
+    u = (draws >> 8).astype(np.float64) / 2**24
+    r = np.corrcoef(u[:-2], u[2:])[0, 1]
+    assert abs(r) < 0.02
+
+
+def test_f32_fields_track_the_f64_solver(oracle_lib):
+    # the infection model's two species, 200 steps on a fixed lattice with
+    # secretion and uptake: relative L2 within 1e-5 of the serial f64 solver
+    m, labels, types = case("infection128")[:3]
+    emu = oracle.EmuSimulation(m, labels, types)
+    ser = oracle.SerialSimulation(m, labels, types)
+    for _ in range(200):
+        emu.fields_step()
+        ser.fields_step()
+    for k in range(2):
+        f32 = emu.field_f32(k).astype(np.float64)
+        f64 = ser.field(k)
+        rel = np.linalg.norm(f32 - f64) / np.linalg.norm(f64)
+        assert rel < 1e-5, (k, rel)
+
+
+def spread(v, target):
+    return float(np.sqrt(((v.astype(np.float64) - target) ** 2).mean()))
+
+
+def test_the_schedule_reproduces_the_serial_distribution_over_32_seeds(oracle_lib):
+    # gpu_matches_cpu.rs:48-92 over 32 seeds: same cell count, mean volume
+    # equal (a confluent sheet), spread about target within a quarter
+    sc, se = [], []
+    for seed in range(1, 33):
+        m = ref_model(64, 64, seed=seed)
+        l, t = tiled(m)
+        ser = oracle.SerialSimulation(m, l, t)
+        emu = oracle.EmuSimulation(m, l, t)
+        ser.step(200)
+        emu.step(200)
+        vs, ve = ser.volumes()[1:], emu.volumes()[1:]
+        assert len(vs) == len(ve)
+        assert abs(vs.mean() - ve.mean()) < 1e-9
+        assert (ve >= 1).all()
+        sc.append(spread(vs, 64.0))
+        se.append(spread(ve, 64.0))
+    a, b = np.mean(sc), np.mean(se)
+    assert abs(a - b) / max(a, b) < 0.25, (a, b)
