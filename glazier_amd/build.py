@@ -15,6 +15,8 @@ from pathlib import Path
 HERE = Path(__file__).resolve().parent
 CSRC = HERE / "csrc"
 OUT = HERE / "libglazier_hip.so"
+CLI = HERE / "glazier"
+CLI_SOURCES = ["cli.cpp", "blueprint.cpp"]
 SOURCES = ["sweep.hip", "sweep_plain.hip", "fields.hip", "cells.hip", "engine.cpp"]
 FLAGS = [
     "--offload-arch=gfx950",
@@ -44,6 +46,15 @@ def build(verbose: bool = False) -> Path:
     if verbose and done.stderr:
         print(done.stderr)
     os.replace(tmp, OUT)
+    # the command line, linked against the library beside it
+    tmp_cli = CLI.with_name("glazier.tmp")
+    cmd = [hipcc(), "-O2", "-std=c++17", "-ffp-contract=off", "-Wno-unused-command-line-argument",
+           "-o", str(tmp_cli), *[str(CSRC / s) for s in CLI_SOURCES], "-L", str(HERE),
+           "-lglazier_hip", "-Wl,-rpath,$ORIGIN"]
+    done = subprocess.run(cmd, capture_output=True, text=True)
+    if done.returncode != 0:
+        raise RuntimeError("hipcc failed on the command line:\n" + done.stderr[-4000:])
+    os.replace(tmp_cli, CLI)
     return OUT
 
 

@@ -434,6 +434,9 @@ ghip_status run_sweep_pass(ghip_sim *s, int colour) {
         pp.tgt_s = s->host_tables.tgt_s[1];
         pp.thr = s->tables->thr;
         pp.n_thr = s->host_tables.n_thr;
+        // with both contact rows flat across the medium and the cell type, a
+        // medium neighbour prices like a cell one and the medium count drops out
+        pp.needs_medium = !(pp.j[0][0] == pp.j[0][1] && pp.j[1][0] == pp.j[1][1]);
         HIP_TRY(launch_sweep_plain2d(a, pp, s->blocks, s->stream));
     } else {
         HIP_TRY(launch_sweep(a, s->d3, s->full, s->blocks, s->threads, s->stream));

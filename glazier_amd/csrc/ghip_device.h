@@ -88,6 +88,7 @@ struct PlainParams {
     float lam_v, tgt_v, lam_s, tgt_s;
     const uint32_t *thr;    // integral acceptance table
     uint32_t n_thr;
+    uint32_t needs_medium;  // whether a medium neighbour can change a price
 };
 
 // One launch of the field step: `steps` substeps of one species.

@@ -72,21 +72,25 @@ __global__ __launch_bounds__(256) void sweep_plain2d_kernel(SweepArgs a, PlainPa
         delta_s[tid] = dy * kPitch + dx;
     }
 
-    // ---- issue every global read of the workgroup before the first wait
+    // ---- issue every global read of the workgroup before the first wait.
+    // A wavefront loads two tile rows per step: lanes 0-31 the first, lanes
+    // 32-63 the second; the row offsets are scalars chosen per half-wave.
+    const int half = lane >> 5;
+    const uint32_t col4 = (uint32_t)(lane & 31) * 4u;
     uint4 v[3];
     int vdst[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        // row r of the tile: 8k + 2*wave + (lane >> 5); column group lane & 31
-        const int r = 8 * k + 2 * wave + (lane >> 5);
+        const int r0 = 8 * k + 2 * wave;  // uniform
         vdst[k] = -1;
-        if (r < kRows) {
-            int gy = y0 - 1 + r;
-            gy += gy < 0 ? H : 0;
-            gy -= gy >= H ? H : 0;
-            const uint32_t off = (uint32_t)gy * (uint32_t)W + (uint32_t)(x0 + 4 * (lane & 31));
-            v[k] = *reinterpret_cast<const uint4 *>(a.labels + off);
-            vdst[k] = r * kPitch + 4 + 4 * (lane & 31);
+        if (r0 < kRows) {
+            int g0 = y0 - 1 + r0, g1 = g0 + 1;
+            g0 += g0 < 0 ? H : 0;
+            g0 -= g0 >= H ? H : 0;
+            g1 -= g1 >= H ? H : 0;
+            const uint32_t row = (uint32_t)(half ? g1 : g0) * (uint32_t)W + (uint32_t)x0;
+            v[k] = *reinterpret_cast<const uint4 *>(a.labels + row + col4);
+            vdst[k] = (r0 + half) * kPitch + 4 + (int)col4;
         }
     }
     uint32_t hv = 0;
@@ -120,7 +124,7 @@ __global__ __launch_bounds__(256) void sweep_plain2d_kernel(SweepArgs a, PlainPa
     for (int k = 0; k < 3; ++k) {
         if (vdst[k] >= 0) {
             *reinterpret_cast<uint4 *>(tile + vdst[k]) = v[k];
-            lo = min(lo, min(min(v[k].x, v[k].y), min(v[k].z, v[k].w)));
+            if (p.needs_medium) lo = min(lo, min(min(v[k].x, v[k].y), min(v[k].z, v[k].w)));
         }
     }
     if (hpos >= 0) {
@@ -136,7 +140,7 @@ __global__ __launch_bounds__(256) void sweep_plain2d_kernel(SweepArgs a, PlainPa
         a.a_next[i] = a.a_cur[i] + a.e_prev[i];
         a.g_zero[i] = 0;
     }
-    const bool has_medium = __syncthreads_or(lo == 0u) != 0;
+    const bool has_medium = __syncthreads_or(p.needs_medium && lo == 0u) != 0;
 
     // ---- phase A: draw every target, queue the ones whose label would change
 #pragma unroll
@@ -194,10 +198,14 @@ __global__ __launch_bounds__(256) void sweep_plain2d_kernel(SweepArgs a, PlainPa
         if (has_medium) med = 8 - (int)unequal8(0u, n0, n1, n2, n3, n4, n5, n6, n7);
         // contact gained minus lost, from counts (exact: integer energies)
         const int etn = nid != 0 ? 1 : 0, eto = oid != 0 ? 1 : 0;
+        // neighbours unlike the new label: 8 - like_new; the medium ones among
+        // them are all medium neighbours when the new label is a cell, none
+        // when it is the medium (then like_new counts them). So `med` enters
+        // only beside a cell label, where flat contact rows cancel it.
         const int med_ne_new = nid != 0 ? med : 0;
-        const int cell_ne_new = (8 - med) - (nid != 0 ? like_new : 0);
+        const int cell_ne_new = unlike_new - med_ne_new;
         const int med_ne_old = oid != 0 ? med : 0;
-        const int cell_ne_old = (8 - med) - (oid != 0 ? like_old : 0);
+        const int cell_ne_old = unlike_old - med_ne_old;
         const float cn = p.j[etn][0] * (float)med_ne_new + p.j[etn][1] * (float)cell_ne_new;
         const float co = p.j[eto][0] * (float)med_ne_old + p.j[eto][1] * (float)cell_ne_old;
         float dh = cn - co;

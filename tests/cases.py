@@ -53,6 +53,22 @@ def case(name):
     if name == "surface1024":
         m, l, t = plain_monolayer(1024, 3, True)
         return m, l, t, None, None, 3
+    if name == "fast_medium_flat":
+        # plain fast path with medium gaps; the medium prices as type 1
+        m = ref_model(128, 128, seed=12)
+        l, t = tiled(m, side=8, nx=12, ny=12)
+        return m, l, t, None, None, 15
+    if name == "fast_medium_typed":
+        # plain fast path where the medium prices as the medium (cell_type[0] = 0)
+        m = ref_model(128, 128, seed=13)
+        l, t = tiled(m, side=8, nx=12, ny=12)
+        t[0] = 0
+        return m, l, t, None, None, 15
+    if name == "fast_lone_typed":
+        m = ref_model(256, 128, seed=14)
+        l, t = lone_cell(m, 40, 48)
+        t[0] = 0
+        return m, l, t, None, None, 20
     if name == "ref64":
         m = ref_model(seed=5)
         return (m, *tiled(m), None, None, 30)
@@ -139,7 +155,8 @@ def case(name):
     raise KeyError(name)
 
 
-PARITY_CASES = ["plain128", "surface256", "surface1024", "ref64", "medium_gaps", "rect64x32",
+PARITY_CASES = ["plain128", "surface256", "surface1024", "fast_medium_flat", "fast_medium_typed",
+                "fast_lone_typed", "ref64", "medium_gaps", "rect64x32",
                 "von_neumann", "float_energies", "volume3d_o2", "volume3d_o3", "volume3d_o1",
                 "connected", "connected3d", "chemotaxis_ramp", "secreting", "length",
                 "motility", "drift", "death", "division", "infection128", "motility3d"]
