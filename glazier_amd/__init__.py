@@ -7,13 +7,14 @@ and Blueprint reader, and the ctypes binding of the engine.
 """
 
 from .model import CellType, Model, Species, default_cell_types, tiled_labels
-from .engine import GhipError, HipSimulation
+from .engine import GhipError, HipBatch, HipSimulation
 
 __all__ = [
     "CellType",
     "Model",
     "Species",
     "GhipError",
+    "HipBatch",
     "HipSimulation",
     "default_cell_types",
     "tiled_labels",

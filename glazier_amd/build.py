@@ -10,6 +10,7 @@ from __future__ import annotations
 
 import os
 import subprocess
+from concurrent.futures import ThreadPoolExecutor
 from pathlib import Path
 
 HERE = Path(__file__).resolve().parent
@@ -17,7 +18,9 @@ CSRC = HERE / "csrc"
 OUT = HERE / "libglazier_hip.so"
 CLI = HERE / "glazier"
 CLI_SOURCES = ["cli.cpp", "blueprint.cpp"]
-SOURCES = ["sweep.hip", "sweep_plain.hip", "fields.hip", "cells.hip", "engine.cpp"]
+SOURCES = ["sweep.hip", "sweep_plain.hip", "fields.hip", "cells.hip", "batch.hip", "engine.cpp",
+           "batch_host.cpp"]
+OBJ_DIR = HERE / "_build"
 FLAGS = [
     "--offload-arch=gfx950",
     "-O3",
@@ -37,12 +40,27 @@ def hipcc() -> str:
     return "hipcc"
 
 
-def build(verbose: bool = False) -> Path:
-    tmp = OUT.with_suffix(".so.tmp")
-    cmd = [hipcc(), *FLAGS, "-o", str(tmp), *[str(CSRC / s) for s in SOURCES]]
+def _compile(src: str) -> Path:
+    obj = OBJ_DIR / (src.replace(".", "_") + ".o")
+    flags = [f for f in FLAGS if f != "-shared"]
+    cmd = [hipcc(), *flags, "-c", "-o", str(obj), str(CSRC / src)]
     done = subprocess.run(cmd, capture_output=True, text=True)
     if done.returncode != 0:
-        raise RuntimeError("hipcc failed:\n" + done.stderr[-4000:])
+        raise RuntimeError(f"hipcc failed on {src}:\n" + done.stderr[-4000:])
+    return obj
+
+
+def build(verbose: bool = False) -> Path:
+    OBJ_DIR.mkdir(exist_ok=True)
+    # one translation unit per process: the kernel files dominate the time
+    workers = max(1, min(len(SOURCES), os.cpu_count() or 1, 8))
+    with ThreadPoolExecutor(max_workers=workers) as pool:
+        objects = list(pool.map(_compile, SOURCES))
+    tmp = OUT.with_suffix(".so.tmp")
+    cmd = [hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", "-o", str(tmp), *map(str, objects)]
+    done = subprocess.run(cmd, capture_output=True, text=True)
+    if done.returncode != 0:
+        raise RuntimeError("linking failed:\n" + done.stderr[-4000:])
     if verbose and done.stderr:
         print(done.stderr)
     os.replace(tmp, OUT)

@@ -1,0 +1,411 @@
+// cell_body.h - per-label bookkeeping on gfx950: recounts of volume, surface
+// and integer moment sums, anchors, and division/death with label
+// compaction, all on the device with no host round trip.
+//
+// Device counterpart of recount / cell_circular_sums / cell_centroids /
+// cell_sums / cell_moments / apply_population (glazier-cuda/src/lib.rs:66-172,
+// 255-310) and of the host decisions in run_population (lib.rs:1362-1461),
+// which the reference takes through three blocking device->host copies per
+// step. Serial counterparts: count_surface, count_moments, divide_and_die
+// (glazier-core/src/cpu.rs:11-59, 620-725).
+//
+// Per-label state is `cw` int64 words: word 0 packs volume * 2^32 + surface;
+// words 1..9 (when moments are kept) are the sums of ux, uy, uz, ux^2, uy^2,
+// uz^2, ux*uy, ux*uz, uy*uz over the label's sites, u being the site's
+// minimum-image offset from the label's integer anchor. Integer sums are
+// exact, so their order of accumulation does not matter.
+#pragma once
+
+#include "ghip_device.h"
+#include "geometry.h"
+
+namespace ghip {
+
+namespace {
+__device__ __forceinline__ void coords(size_t i, int w, int h, int *x, int *y, int *z) {
+    size_t plane = (size_t)w * h;
+    *z = (int)(i / plane);
+    size_t r = i % plane;
+    *y = (int)(r / w);
+    *x = (int)(r % w);
+}
+}  // namespace
+
+__device__ __forceinline__ void fill_i64_body(int64_t *p, size_t n, const int lb, const int gb) {
+    size_t i = (size_t)lb * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+__device__ __forceinline__ void fill_u32_body(uint32_t *p, size_t n, uint32_t v, const int lb, const int gb) {
+    size_t i = (size_t)lb * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// a += e; e = 0, over the words of the labels in use.
+__device__ __forceinline__ void consolidate_body(int64_t *a, int64_t *e, const uint32_t *n_labels, int cw, const int lb, const int gb) {
+    size_t n = (size_t)(*n_labels) * cw;
+    for (size_t i = (size_t)lb * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gb * blockDim.x) {
+        a[i] += e[i];
+        e[i] = 0;
+    }
+}
+
+// Lowest site index of every label.
+__device__ __forceinline__ void first_site_body(CellArgs a, size_t n_sites, const int lb, const int gb) {
+    size_t i = (size_t)lb * blockDim.x + threadIdx.x;
+    if (i >= n_sites) return;
+    atomicMin(a.scratch_u32 + label_id(a.labels[i]), (uint32_t)i);
+}
+
+__device__ __forceinline__ void anchor_from_first_body(CellArgs a, const int lb, const int gb) {
+    uint32_t l = lb * blockDim.x + threadIdx.x;
+    if (l >= *a.n_labels) return;
+    uint32_t f = a.scratch_u32[l];
+    int x = 0, y = 0, z = 0;
+    if (f != 0xFFFFFFFFu) coords(f, a.w, a.h, &x, &y, &z);
+    a.anchor[3 * l] = x;
+    a.anchor[3 * l + 1] = y;
+    a.anchor[3 * l + 2] = z;
+}
+
+// Recount of every label from the lattice into a zeroed state (or, with
+// moments_only, of the moment words alone). A workgroup pre-aggregates its
+// sites per label in an LDS hash table with LDS atomics, then flushes one
+// global atomic per (label, word); labels that find no free slot go straight
+// to global atomics. `gate` >= 0 makes the launch a no-op unless
+// flags[gate] is set, so a whole step can be enqueued without host checks.
+constexpr int kSlots = 256;
+
+__device__ __forceinline__ void recount_body(CellArgs a, size_t n_sites, int sites_per_block,
+                                                      int moments_only, int gate, const int lb, const int gb) {
+    if (gate >= 0 && a.flags[gate] == 0) return;
+    __shared__ uint32_t keys[kSlots];
+    __shared__ unsigned long long vals[kSlots * 10];
+    const int cw = a.cw;
+    for (int s = threadIdx.x; s < kSlots; s += blockDim.x) keys[s] = 0xFFFFFFFFu;
+    for (int s = threadIdx.x; s < kSlots * cw; s += blockDim.x) vals[s] = 0;
+    __syncthreads();
+    size_t lo = (size_t)lb * sites_per_block;
+    size_t hi = lo + sites_per_block;
+    if (hi > n_sites) hi = n_sites;
+    for (size_t i = lo + threadIdx.x; i < hi; i += blockDim.x) {
+        uint32_t word = a.labels[i];
+        uint32_t l = label_id(word);
+        int x, y, z;
+        coords(i, a.w, a.h, &x, &y, &z);
+        int64_t c[10];
+        for (int q = 0; q < 10; ++q) c[q] = 0;
+        if (!moments_only) {
+            int bonds = 0;
+            for (int k = 0; k < a.n_off; ++k) {
+                size_t nb = ((size_t)wrapc(z + a.off[k][2], a.d) * a.h + wrapc(y + a.off[k][1], a.h)) *
+                                (size_t)a.w +
+                            wrapc(x + a.off[k][0], a.w);
+                if (nb != i && a.labels[nb] != word) ++bonds;
+            }
+            c[0] = pack_vs(1, bonds);
+        }
+        if (cw > 1 && l != 0) {
+            int64_t ux = fold_int(x - a.anchor[3 * l], a.w);
+            int64_t uy = fold_int(y - a.anchor[3 * l + 1], a.h);
+            int64_t uz = fold_int(z - a.anchor[3 * l + 2], a.d);
+            c[1] = ux;
+            c[2] = uy;
+            c[3] = uz;
+            c[4] = ux * ux;
+            c[5] = uy * uy;
+            c[6] = uz * uz;
+            c[7] = ux * uy;
+            c[8] = ux * uz;
+            c[9] = uy * uz;
+        }
+        uint32_t slot = hash32(l) & (kSlots - 1);
+        int found = -1;
+        for (int probe = 0; probe < kSlots; ++probe) {
+            uint32_t prev = atomicCAS(&keys[slot], 0xFFFFFFFFu, l);
+            if (prev == 0xFFFFFFFFu || prev == l) {
+                found = (int)slot;
+                break;
+            }
+            slot = (slot + 1) & (kSlots - 1);
+        }
+        for (int q = 0; q < cw; ++q) {
+            if (c[q] == 0) continue;
+            if (found >= 0)
+                atomicAdd(&vals[found * cw + q], (unsigned long long)c[q]);
+            else
+                atomicAdd(reinterpret_cast<unsigned long long *>(a.state + (size_t)l * cw + q),
+                          (unsigned long long)c[q]);
+        }
+    }
+    __syncthreads();
+    for (int s = threadIdx.x; s < kSlots * cw; s += blockDim.x) {
+        uint32_t l = keys[s / cw];
+        if (l == 0xFFFFFFFFu || vals[s] == 0) continue;
+        atomicAdd(reinterpret_cast<unsigned long long *>(a.state + (size_t)l * cw + s % cw), vals[s]);
+    }
+}
+
+// Zero the moment words (1..9) of every label, keeping volume and surface.
+__device__ __forceinline__ void zero_moments_body(CellArgs a, const int lb, const int gb) {
+    uint32_t l = lb * blockDim.x + threadIdx.x;
+    if (l >= *a.n_labels) return;
+    for (int q = 1; q < a.cw; ++q) a.state[(size_t)l * a.cw + q] = 0;
+}
+
+// Move each live label's anchor to its rounded centroid, then clear its sums
+// for a moments-only recount. Gated on flags[gate] when gate >= 0.
+__device__ __forceinline__ void recentre_body(CellArgs a, int gate, const int lb, const int gb) {
+    if (gate >= 0 && a.flags[gate] == 0) return;
+    uint32_t l = lb * blockDim.x + threadIdx.x;
+    if (l >= *a.n_labels) return;
+    int64_t *s = a.state + (size_t)l * a.cw;
+    int64_t v = unpack_vol(s[0]);
+    if (l != 0 && v > 0) {
+        const int span[3] = {a.w, a.h, a.d};
+        for (int ax = 0; ax < 3; ++ax) {
+            double mean = (double)s[1 + ax] / (double)v;
+            int64_t shift = (int64_t)floor(mean + 0.5);
+            int64_t nv = (int64_t)a.anchor[3 * l + ax] + shift;
+            nv %= span[ax];
+            if (nv < 0) nv += span[ax];
+            a.anchor[3 * l + ax] = (int32_t)nv;
+        }
+    }
+    for (int q = 1; q < a.cw; ++q) s[q] = 0;
+}
+
+// Whether some live label's centroid sits more than an eighth of a side from
+// its anchor; sets flags[1].
+__device__ __forceinline__ void check_recentre_body(CellArgs a, const int lb, const int gb) {
+    uint32_t l = lb * blockDim.x + threadIdx.x;
+    if (l == 0 || l >= *a.n_labels) return;
+    const int64_t *s = a.state + (size_t)l * a.cw;
+    int64_t n = unpack_vol(s[0]);
+    if (n <= 0) return;
+    const int span[3] = {a.w, a.h, a.d};
+    for (int ax = 0; ax < 3; ++ax) {
+        if (span[ax] <= 1) continue;
+        int64_t m = s[1 + ax];
+        int64_t am = m < 0 ? -m : m;
+        if (8 * am > (int64_t)span[ax] * n) {
+            a.flags[1] = 1;
+            return;
+        }
+    }
+}
+
+// ---------------------------------------------------------------- population
+
+// Decide each live label's fate: divide at its division volume (cut through
+// the centroid across the major axis), else die with the type's rate by a
+// counter draw of (step, label). Sets flags[0] when anything happens.
+__device__ __forceinline__ void plan_body(CellArgs a, const int lb, const int gb) {
+    uint32_t l = lb * blockDim.x + threadIdx.x;
+    uint32_t n = *a.n_labels;
+    if (l >= n) return;
+    a.plan[l] = 0;
+    if (l == 0) return;
+    const int64_t *s = a.state + (size_t)l * a.cw;
+    int64_t v = unpack_vol(s[0]);
+    if (v <= 0) return;
+    uint32_t t = label_type(a.ltype[l]);
+    const DeviceTables *tab = a.tables;
+    if (tab->div_vol[t] > 0.0 && (double)v >= tab->div_vol[t] && v >= 4) {
+        a.plan[l] = 1;
+        double *g = a.geo + 6 * (size_t)l;
+        for (int ax = 0; ax < 3; ++ax) g[ax] = (double)s[1 + ax] / (double)v;
+        cov_largest(s + 1, v, a.d > 1, g + 3);
+        a.flags[0] = 1;
+    } else if (tab->death[t] > 0.0) {
+        uint32_t hsh = hash32(hash32(l ^ a.key) ^ a.key2);
+        if ((double)hsh * (1.0 / 4294967296.0) < tab->death[t]) {
+            a.plan[l] = 2;
+            a.flags[0] = 1;
+        }
+    }
+}
+
+__device__ __forceinline__ bool beyond_cut(const CellArgs &a, uint32_t l, int x, int y, int z) {
+    double ux = (double)fold_int(x - a.anchor[3 * l], a.w);
+    double uy = (double)fold_int(y - a.anchor[3 * l + 1], a.h);
+    double uz = (double)fold_int(z - a.anchor[3 * l + 2], a.d);
+    const double *g = a.geo + 6 * (size_t)l;
+    double along = (ux - g[0]) * g[3] + (uy - g[1]) * g[4] + (uz - g[2]) * g[5];
+    return along > 0.0;
+}
+
+// Sites each planned division would move.
+__device__ __forceinline__ void count_moved_body(CellArgs a, size_t n_sites, const int lb, const int gb) {
+    if (a.flags[0] == 0) return;
+    size_t i = (size_t)lb * blockDim.x + threadIdx.x;
+    if (i >= n_sites) return;
+    uint32_t l = label_id(a.labels[i]);
+    if (l == 0 || a.plan[l] != 1) return;
+    int x, y, z;
+    coords(i, a.w, a.h, &x, &y, &z);
+    if (beyond_cut(a, l, x, y, z)) atomicAdd(a.scratch_u32 + l, 1u);
+}
+
+// One workgroup: cancel divisions that would move nothing or everything,
+// hand out daughter ids in ascending parent order, and copy types and
+// anchors to the daughters.
+__device__ __forceinline__ void assign_daughters_body(CellArgs a, const int lb, const int gb) {
+    if (a.flags[0] == 0) return;
+    __shared__ uint32_t part[1024];
+    __shared__ uint32_t base;
+    const uint32_t n0 = *a.n_labels;
+    if (threadIdx.x == 0) base = n0;
+    __syncthreads();
+    for (uint32_t start = 0; start < n0; start += blockDim.x) {
+        uint32_t l = start + threadIdx.x;
+        uint32_t take = 0;
+        if (l < n0 && l != 0 && a.plan[l] == 1) {
+            int64_t v = unpack_vol(a.state[(size_t)l * a.cw]);
+            uint32_t moved = a.scratch_u32[l];
+            if (moved == 0 || (int64_t)moved == v)
+                a.plan[l] = 0;
+            else
+                take = 1;
+        }
+        part[threadIdx.x] = take;
+        __syncthreads();
+        for (int off = 1; off < (int)blockDim.x; off <<= 1) {
+            uint32_t add = threadIdx.x >= (unsigned)off ? part[threadIdx.x - off] : 0;
+            __syncthreads();
+            part[threadIdx.x] += add;
+            __syncthreads();
+        }
+        if (take) {
+            uint32_t id = base + part[threadIdx.x] - 1;
+            if (id >= a.capacity) {
+                a.plan[l] = 0;
+                a.flags[2] = 1;
+            } else {
+                a.ltype[id] = (a.ltype[l] & ~kIdMask) | id; // the daughter keeps the type
+                a.daughter[l] = id;
+                a.anchor[3 * id] = a.anchor[3 * l];
+                a.anchor[3 * id + 1] = a.anchor[3 * l + 1];
+                a.anchor[3 * id + 2] = a.anchor[3 * l + 2];
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x == blockDim.x - 1) base += part[threadIdx.x];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *a.n_labels = base < a.capacity ? base : a.capacity;
+}
+
+// Relabel: a dying label's sites become medium, a dividing label's sites
+// beyond the cut go to its daughter.
+__device__ __forceinline__ void apply_population_body(CellArgs a, size_t n_sites, const int lb, const int gb) {
+    if (a.flags[0] == 0) return;
+    size_t i = (size_t)lb * blockDim.x + threadIdx.x;
+    if (i >= n_sites) return;
+    uint32_t word = a.labels[i];
+    uint32_t l = label_id(word);
+    if (l == 0) return;
+    uint8_t p = a.plan[l];
+    if (p == 0) return;
+    if (p == 2) {
+        a.labels[i] = 0;
+        return;
+    }
+    int x, y, z;
+    coords(i, a.w, a.h, &x, &y, &z);
+    if (beyond_cut(a, l, x, y, z)) {
+        a.labels[i] = (word & ~kIdMask) | a.daughter[l];
+    }
+}
+
+// Zero the per-label state ahead of a full recount, gated like recount.
+__device__ __forceinline__ void zero_state_body(CellArgs a, int gate, const int lb, const int gb) {
+    if (gate >= 0 && a.flags[gate] == 0) return;
+    size_t n = (size_t)(*a.n_labels) * a.cw;
+    for (size_t i = (size_t)lb * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gb * blockDim.x)
+        a.state[i] = 0;
+}
+
+// One workgroup: when more than max(64, (n-1)/4) labels are dead, renumber
+// the live ones densely in ascending order (remap into scratch_u32) and set
+// flags[3].
+__device__ __forceinline__ void compact_plan_body(CellArgs a, const int lb, const int gb) {
+    if (a.flags[0] == 0) return;
+    __shared__ uint32_t part[1024];
+    __shared__ uint32_t dead_total;
+    __shared__ uint32_t base;
+    const uint32_t n = *a.n_labels;
+    if (threadIdx.x == 0) {
+        dead_total = 0;
+        base = 1;
+    }
+    __syncthreads();
+    for (uint32_t l = 1 + threadIdx.x; l < n; l += blockDim.x)
+        if (unpack_vol(a.state[(size_t)l * a.cw]) <= 0) atomicAdd(&dead_total, 1u);
+    __syncthreads();
+    uint32_t limit = (n - 1) / 4;
+    if (limit < 64) limit = 64;
+    if (dead_total <= limit) return;
+    for (uint32_t start = 1; start < n; start += blockDim.x) {
+        uint32_t l = start + threadIdx.x;
+        uint32_t live = (l < n && unpack_vol(a.state[(size_t)l * a.cw]) > 0) ? 1u : 0u;
+        part[threadIdx.x] = live;
+        __syncthreads();
+        for (int off = 1; off < (int)blockDim.x; off <<= 1) {
+            uint32_t add = threadIdx.x >= (unsigned)off ? part[threadIdx.x - off] : 0;
+            __syncthreads();
+            part[threadIdx.x] += add;
+            __syncthreads();
+        }
+        if (l < n) a.scratch_u32[l] = live ? base + part[threadIdx.x] - 1 : 0u;
+        __syncthreads();
+        if (threadIdx.x == blockDim.x - 1) base += part[threadIdx.x];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        a.scratch_u32[0] = 0;
+        a.flags[3] = base; // the new label count, nonzero
+    }
+}
+
+// Move types and anchors of live labels to their new ids (ascending, so a
+// label only ever moves down and no source is overwritten before it is read
+// when done in one thread; the parallel form reads from a staged copy).
+__device__ __forceinline__ void compact_move_body(CellArgs a, const uint32_t *type_copy, const int32_t *anchor_copy, const int lb, const int gb) {
+    if (a.flags[0] == 0 || a.flags[3] == 0) return;
+    uint32_t l = lb * blockDim.x + threadIdx.x;
+    uint32_t n = *a.n_labels;
+    if (l == 0 || l >= n) return;
+    uint32_t to = a.scratch_u32[l];
+    if (to == 0) return;
+    a.ltype[to] = (type_copy[l] & ~kIdMask) | to;
+    a.anchor[3 * to] = anchor_copy[3 * l];
+    a.anchor[3 * to + 1] = anchor_copy[3 * l + 1];
+    a.anchor[3 * to + 2] = anchor_copy[3 * l + 2];
+}
+
+__device__ __forceinline__ void compact_relabel_body(CellArgs a, size_t n_sites, const int lb, const int gb) {
+    if (a.flags[0] == 0 || a.flags[3] == 0) return;
+    size_t i = (size_t)lb * blockDim.x + threadIdx.x;
+    if (i >= n_sites) return;
+    uint32_t word = a.labels[i];
+    uint32_t l = label_id(word);
+    if (l == 0) return;
+    a.labels[i] = (word & ~kIdMask) | a.scratch_u32[l];
+}
+
+__device__ __forceinline__ void compact_finish_body(CellArgs a, const int lb, const int gb) {
+    if (a.flags[0] == 0 || a.flags[3] == 0) return;
+    if (lb == 0 && threadIdx.x == 0) *a.n_labels = a.flags[3];
+}
+
+__device__ __forceinline__ void copy_u32_body(uint32_t *dst, const uint32_t *src, const uint32_t *n_labels, int per, const int lb, const int gb) {
+    size_t n = (size_t)(*n_labels) * per;
+    for (size_t i = (size_t)lb * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gb * blockDim.x)
+        dst[i] = src[i];
+}
+
+}  // namespace ghip

@@ -18,6 +18,7 @@
 
 #include "../../include/glazier_hip.h"
 #include "ghip_device.h"
+#include "sim.h"
 
 using namespace ghip;
 
@@ -25,167 +26,18 @@ namespace {
 
 thread_local std::string g_last_error;
 
-ghip_status fail(ghip_status code, const std::string &msg) {
-    g_last_error = msg;
-    return code;
-}
-
-#define HIP_TRY(expr)                                                                          \
-    do {                                                                                       \
-        hipError_t _e = (expr);                                                                \
-        if (_e != hipSuccess)                                                                  \
-            return fail(GHIP_DEVICE, std::string(#expr) + ": " + hipGetErrorString(_e));       \
-    } while (0)
-
-int ilog2(int v) {
-    int r = 0;
-    while ((1 << (r + 1)) <= v) ++r;
-    return r;
-}
-
-int pow2_floor(int v) { return v < 1 ? 1 : 1 << ilog2(v); }
-
 bool is_integer(double v) { return v == std::floor(v) && std::fabs(v) < 1048576.0; }
-
-struct TimedLaunch {
-    int family;
-    hipEvent_t start, stop;
-};
 
 const char *kFamilies[] = {"sweep", "fields", "moments", "population"};
 
 }  // namespace
 
-struct ghip_sim {
-    ghip_model m{};
-    int W = 0, H = 0, D = 0;
-    size_t N = 0;
-    bool d3 = false;
-    int colours = 4;
-    int n_off = 0;
-    int off[26][3] = {};
-    hipStream_t stream = nullptr;
-
-    uint32_t *labels[2] = {nullptr, nullptr};
-    int lab_cur = 0;
-    float *act[2] = {nullptr, nullptr};
-    int act_cur = 0;
-    float *field[GHIP_MAX_SPECIES][2] = {};
-    float *field_base[2] = {nullptr, nullptr};
-    int field_cur[GHIP_MAX_SPECIES] = {};
-    int64_t *A[2] = {nullptr, nullptr};
-    int64_t *Dl[3] = {nullptr, nullptr, nullptr};
-    int cw = 1;
-    uint32_t cap = 0;
-    int32_t *anchor = nullptr;
-    uint32_t *ltype = nullptr;
-    uint32_t *scratch = nullptr;
-    uint32_t *daughter = nullptr;
-    uint8_t *plan = nullptr;
-    double *geo = nullptr;
-    uint32_t *flags = nullptr;
-    uint32_t *type_copy = nullptr;
-    int32_t *anchor_copy = nullptr;
-    uint32_t *n_labels_dev = nullptr;
-    DeviceTables *tables = nullptr;
-    DeviceTables host_tables{};
-    unsigned long long *accepted = nullptr;
-
-    uint64_t pass = 0;
-    uint64_t mcs = 0;
-    uint64_t attempted = 0;
-    uint32_t sweep_flags = 0;
-    bool full = false;
-    bool has_pop = false, tracks_moments = false, has_mot = false;
-    int substeps[GHIP_MAX_SPECIES] = {};
-    float fd[GHIP_MAX_SPECIES] = {}, fk[GHIP_MAX_SPECIES] = {};
-    int medium_type = 0;
-
-    int tx_log2 = 6, ty_log2 = 3, tz_log2 = 0, threads = 512;
-    int tiles_x = 1, tiles_y = 1, tiles_z = 1, blocks = 1;
-    bool plain_fast = false;  // the 128 x 16 plain-monolayer pass applies
-    int fx_log2 = 6, fy_log2 = 5, fz = 1, f_tiles_x = 1, f_tiles_y = 1, f_blocks = 1, f_kmax = 4;
-
-    bool timing = false;
-    std::vector<TimedLaunch> pending;
-    std::vector<TimedLaunch> spare;
-    double ms[4] = {0, 0, 0, 0};
-    uint64_t launches[4] = {0, 0, 0, 0};
-
-    ~ghip_sim() {
-        if (stream) (void)hipStreamSynchronize(stream);
-        void *bufs[] = {labels[0], labels[1], act[0], act[1], field_base[0], field_base[1], A[0],
-                        A[1], Dl[0], Dl[1], Dl[2], anchor, ltype, scratch, daughter, plan, geo,
-                        flags, type_copy, anchor_copy, n_labels_dev, tables, accepted};
-        for (void *p : bufs)
-            if (p) (void)hipFree(p);
-        for (auto *list : {&pending, &spare})
-            for (auto &t : *list) {
-                (void)hipEventDestroy(t.start);
-                (void)hipEventDestroy(t.stop);
-            }
-        if (stream) (void)hipStreamDestroy(stream);
-    }
-
-    CellArgs cell_args() {
-        CellArgs c{};
-        c.labels = labels[lab_cur];
-        c.state = A[pass % 2];
-        c.anchor = anchor;
-        c.n_labels = n_labels_dev;
-        c.ltype = ltype;
-        c.scratch_u32 = scratch;
-        c.daughter = daughter;
-        c.plan = plan;
-        c.geo = geo;
-        c.flags = flags;
-        c.tables = tables;
-        c.w = W;
-        c.h = H;
-        c.d = D;
-        c.cw = cw;
-        c.n_off = n_off;
-        std::memcpy(c.off, off, sizeof off);
-        c.capacity = cap;
-        return c;
-    }
-
-    hipError_t begin_timed(int family, TimedLaunch *out) {
-        if (!timing) return hipSuccess;
-        if (spare.empty()) {
-            TimedLaunch t{};
-            hipError_t e = hipEventCreate(&t.start);
-            if (e != hipSuccess) return e;
-            e = hipEventCreate(&t.stop);
-            if (e != hipSuccess) return e;
-            spare.push_back(t);
-        }
-        *out = spare.back();
-        spare.pop_back();
-        out->family = family;
-        return hipEventRecord(out->start, stream);
-    }
-
-    hipError_t end_timed(TimedLaunch &t) {
-        if (!timing) return hipSuccess;
-        hipError_t e = hipEventRecord(t.stop, stream);
-        pending.push_back(t);
-        return e;
-    }
-
-    hipError_t collect_timing() {
-        for (auto &t : pending) {
-            float elapsed = 0.0f;
-            hipError_t e = hipEventElapsedTime(&elapsed, t.start, t.stop);
-            if (e != hipSuccess) return e;
-            ms[t.family] += elapsed;
-            launches[t.family] += 1;
-            spare.push_back(t);
-        }
-        pending.clear();
-        return hipSuccess;
-    }
-};
+namespace ghip {
+ghip_status fail(ghip_status code, const std::string &msg) {
+    g_last_error = msg;
+    return code;
+}
+}  // namespace ghip
 
 namespace {
 
@@ -373,7 +225,7 @@ void choose_tiles(ghip_sim *s) {
 }
 
 ghip_status run_sweep_pass(ghip_sim *s, int colour) {
-    SweepArgs a{};
+    SweepArgs a = s->sweep_template();
     const uint64_t p = s->pass;
     const bool mot = s->has_mot;
     if (mot) {
@@ -381,11 +233,6 @@ ghip_status run_sweep_pass(ghip_sim *s, int colour) {
         a.labels = s->labels[1 - s->lab_cur];
         a.act_prev = s->act[s->act_cur];
         a.act = s->act[1 - s->act_cur];
-    } else {
-        a.labels = s->labels[s->lab_cur];
-        a.labels_prev = a.labels;
-        a.act = s->act[s->act_cur];
-        a.act_prev = a.act;
     }
     for (uint32_t k = 0; k < s->m.n_species; ++k) a.fields[k] = s->field[k][s->field_cur[k]];
     a.a_cur = s->A[p % 2];
@@ -393,35 +240,14 @@ ghip_status run_sweep_pass(ghip_sim *s, int colour) {
     a.f_acc = s->Dl[p % 3];
     a.a_next = s->A[(p + 1) % 2];
     a.g_zero = s->Dl[(p + 1) % 3];
-    a.anchor = s->anchor;
-    a.tables = s->tables;
-    a.n_labels = s->n_labels_dev;
-    a.accepted = s->accepted;
-    a.w = s->W;
-    a.h = s->H;
-    a.d = s->D;
     a.cx = colour & 1;
     a.cy = (colour >> 1) & 1;
     a.cz = (colour >> 2) & 1;
     a.key = pass_key(s->m.seed, s->m.replica, s->mcs, (uint32_t)colour);
     a.key2 = second_key(a.key);
-    a.tx_log2 = s->tx_log2;
-    a.ty_log2 = s->ty_log2;
-    a.tz_log2 = s->tz_log2;
-    a.halo = mot ? 2 : 1;
-    a.n_off = s->n_off;
-    a.n_types = s->m.n_types;
-    a.n_thr = s->host_tables.n_thr;
-    a.cell_words = s->cw;
-    a.n_species = (int)s->m.n_species;
-    a.flags = s->sweep_flags;
     const uint64_t words = (uint64_t)s->cap * s->cw;
     a.fold_per_block = (uint32_t)((words + s->blocks - 1) / s->blocks);
     a.xcd_chunk = (s->blocks % 8 == 0) ? s->blocks / 8 : 0;
-    a.tiles_x = s->tiles_x;
-    a.tiles_y = s->tiles_y;
-    a.tiles_z = s->tiles_z;
-    std::memcpy(a.off, s->off, sizeof s->off);
     TimedLaunch t{};
     HIP_TRY(s->begin_timed(0, &t));
     if (s->plain_fast) {
@@ -706,6 +532,7 @@ ghip_status ghip_create(const ghip_model *model, const uint32_t *labels, size_t 
 }
 
 ghip_status ghip_destroy(ghip_sim *sim) {
+    if (sim && sim->batch) return fail(GHIP_BAD_ARG, "the lattice is in a batch; destroy the batch first");
     delete sim;
     return GHIP_OK;
 }

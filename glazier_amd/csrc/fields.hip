@@ -1,96 +1,12 @@
-// fields.hip - the chemical-field step on gfx950: explicit-Euler diffusion
-// with decay, temporally blocked in LDS, with secretion and uptake fused into
-// the last substep.
-//
-// Device counterpart of field_diffuse + field_exchange
-// (glazier-cuda/src/lib.rs:314-363, launched per species per substep at
-// lib.rs:1147-1187) and of Fields::diffuse/exchange
-// (glazier-core/src/field.rs:99-149).
-//
-// A workgroup loads an interior tile plus a halo as deep as the substeps it
-// will take, runs those substeps in LDS (each one shrinks the valid region by
-// one site), and writes the interior once. Every site sees the same f32
-// operations in the same order as one substep per launch would give:
-//
-//     sum  = u[x+1] + u[x-1];  sum = sum + u[y+1];  sum = sum + u[y-1]
-//     (3D: sum = sum + u[z+1];  sum = sum + u[z-1])
-//     u'   = (u + d * (sum - faces * u)) - k * u
-//
-// then, on owned sites after the species' last substep:
-//     v = u + sec[type];  v = v - up[type] * v;  u = max(v, 0)
-#include "ghip_device.h"
+// fields.hip - the field step (field_body.h) as a kernel over one lattice and
+// species, the activity decay, and their launchers.
+#include "field_body.h"
 
 namespace ghip {
 
-namespace {
-__device__ __forceinline__ int wrapc(int v, int span) {
-    v %= span;
-    return v < 0 ? v + span : v;
-}
-}  // namespace
-
 template <bool D3>
 __global__ __launch_bounds__(256) void field_kernel(FieldArgs a) {
-    extern __shared__ float fl[];
-    const int IX = 1 << a.ix_log2, IY = 1 << a.iy_log2, IZ = D3 ? a.iz : 1;
-    const int K = a.steps;
-    const int hz = D3 ? K : 0;
-    const int LX = IX + 2 * K, LY = IY + 2 * K, LZ = IZ + 2 * hz;
-    const int total = LX * LY * LZ;
-    float *u = fl;
-    float *v = fl + total;
-    const int b = blockIdx.x;
-    const int tx = b % a.tiles_x, ty = (b / a.tiles_x) % a.tiles_y, tz = D3 ? b / (a.tiles_x * a.tiles_y) : 0;
-    const int x0 = tx * IX - K, y0 = ty * IY - K, z0 = D3 ? tz * IZ - hz : 0;
-
-    for (int q = threadIdx.x; q < total; q += blockDim.x) {
-        int lx = q % LX, ly = (q / LX) % LY, lz = q / (LX * LY);
-        int gx = wrapc(x0 + lx, a.w), gy = wrapc(y0 + ly, a.h), gz = D3 ? wrapc(z0 + lz, a.d) : 0;
-        u[q] = a.in[((size_t)gz * a.h + gy) * (size_t)a.w + gx];
-    }
-    __syncthreads();
-
-    const float faces = D3 ? 6.0f : 4.0f;
-    for (int s = 1; s <= K; ++s) {
-        // valid after s substeps: [s, L - s) on each blocked axis
-        for (int q = threadIdx.x; q < total; q += blockDim.x) {
-            int lx = q % LX, ly = (q / LX) % LY, lz = q / (LX * LY);
-            if (lx < s || lx >= LX - s || ly < s || ly >= LY - s) continue;
-            if (D3 && (lz < s || lz >= LZ - s)) continue;
-            float here = u[q];
-            float sum = u[q + 1] + u[q - 1];
-            sum = sum + u[q + LX];
-            sum = sum + u[q - LX];
-            if (D3) {
-                sum = sum + u[q + LX * LY];
-                sum = sum + u[q - LX * LY];
-            }
-            v[q] = (here + a.dcoef * (sum - faces * here)) - a.kcoef * here;
-        }
-        __syncthreads();
-        float *t = u;
-        u = v;
-        v = t;
-    }
-
-    const int interior = IX * IY * IZ;
-    for (int q = threadIdx.x; q < interior; q += blockDim.x) {
-        int ix = q & (IX - 1), iy = (q >> a.ix_log2) & (IY - 1), iz = q >> (a.ix_log2 + a.iy_log2);
-        int gx = tx * IX + ix, gy = ty * IY + iy, gz = D3 ? tz * IZ + iz : 0;
-        if (gx >= a.w || gy >= a.h || (D3 && gz >= a.d)) continue;
-        size_t site = ((size_t)gz * a.h + gy) * (size_t)a.w + gx;
-        float val = u[((iz + hz) * LY + iy + K) * LX + ix + K];
-        if (a.exchange) {
-            uint32_t word = a.labels[site];
-            if (label_id(word) != 0) {
-                uint32_t t = label_type(word);
-                float w2 = val + a.tables->sec[t][a.species];
-                w2 = w2 - a.tables->up[t][a.species] * w2;
-                val = w2 < 0.0f ? 0.0f : w2;
-            }
-        }
-        a.out[site] = val;
-    }
+    field_body<D3>(a, blockIdx.x);
 }
 
 // Every site's memory steps down by one, floored at zero (activity_decay,

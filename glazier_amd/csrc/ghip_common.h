@@ -50,8 +50,7 @@ GHD uint32_t site_draw(uint32_t key, uint32_t key2, uint32_t site) {
 GHD uint32_t second_uniform(uint32_t h) { return hash32(h ^ 0x68E31DA4U) >> 8; }
 
 // Colour order of one step: a Fisher-Yates shuffle driven by a hash chain.
-inline void colour_order(uint64_t seed, uint32_t replica, uint64_t step, int colours,
-                         int *order) {
+GHD void colour_order(uint64_t seed, uint32_t replica, uint64_t step, int colours, int *order) {
     for (int c = 0; c < colours; ++c) order[c] = c;
     uint32_t h = pass_key(seed, replica, step, 0xC0105EEDU);
     for (int i = colours - 1; i > 0; --i) {
@@ -119,6 +118,12 @@ GHD int fold_int(int d, int span) {
     if (2 * d > span) return d - span;
     if (-2 * d > span) return d + span;
     return d;
+}
+
+// A coordinate taken onto the periodic range [0, span).
+GHD int wrapc(int v, int span) {
+    v %= span;
+    return v < 0 ? v + span : v;
 }
 
 // Label words on the device carry the cell type in their top bits, so the

@@ -1,422 +1,12 @@
-// sweep.hip - one colour pass of the checkerboard Metropolis sweep on gfx950.
-//
-// Device counterpart of cpm_sweep (glazier-cuda/src/lib.rs:436-716) and of
-// the serial attempt (glazier-core/src/cpu.rs:432-517), laid out for CDNA4:
-//
-//  * a workgroup owns a tile of 2TX x 2TY (x 2TZ) sites; it stages the tile's
-//    labels plus a Moore halo in LDS with whole-row loads, so every target
-//    reads its neighbourhood from LDS instead of strided global gathers;
-//  * one lane per target: TX targets along x per row, so a 64-lane wavefront
-//    covers one target row of a 128-site tile row;
-//  * each target takes ONE 32-bit counter hash (pick from the low bits,
-//    uniform from the top 24), so no RNG state is read or written;
-//  * per-label volume/surface/moments are read from the pass-start snapshot
-//    a_cur + e_prev and accepted copies add exact int64 deltas into f_acc,
-//    so the pass is deterministic whatever the wavefront order;
-//  * the same launch folds the snapshot for the next pass
-//    (a_next = a_cur + e_prev) and clears the delta buffer the pass after
-//    next accumulates into, so a pass is one launch with no separate copy.
-//
-// The pass is HBM/Infinity-Cache bound: it must read every label once
-// (4 B per site per colour) and write 4 B per accepted copy.
-#include "ghip_device.h"
-#include "geometry.h"
+// sweep.hip - the general colour pass (sweep_body.h) as a kernel over one
+// lattice, and its launcher.
+#include "sweep_body.h"
 
 namespace ghip {
 
-namespace {
-
-__device__ __forceinline__ int wrap_coord(int v, int span) {
-    if (v < 0) v += span;
-    if (v >= span) v -= span;
-    if (v >= span) v %= span;
-    return v;
-}
-
-// Moore ring of the connectivity test (connectivity.rs:23-29): ORDER3's order,
-// dz then dy then dx from -1 to 1, dropping the z offsets in a plane.
-__device__ __forceinline__ void ring_offset(int k, bool d3, int *dx, int *dy, int *dz) {
-    int q = k;
-    if (d3) {
-        q = k + (k >= 13 ? 1 : 0);
-        *dz = q / 9 - 1;
-        q = q % 9;
-    } else {
-        q = k + (k >= 4 ? 1 : 0);
-        *dz = 0;
-    }
-    *dy = q / 3 - 1;
-    *dx = q % 3 - 1;
-}
-
-// Whether the ring positions set in `present` form one piece.
-__device__ __forceinline__ bool one_piece(uint32_t present, bool d3, const DeviceTables *t) {
-    if (present == 0u || (present & (present - 1u)) == 0u) return true;
-    if (!d3) return (t->piece2d[present >> 5] >> (present & 31u)) & 1u;
-    uint32_t reached = present & (~present + 1u);
-    for (int round = 0; round < 26; ++round) {
-        uint32_t grown = reached;
-        uint32_t rest = reached;
-        while (rest) {
-            int k = __builtin_ctz(rest);
-            grown |= t->adj3d[k];
-            rest &= rest - 1u;
-        }
-        grown &= present;
-        if (grown == reached) break;
-        reached = grown;
-    }
-    return reached == present;
-}
-
-}  // namespace
-
-// FULL adds connectivity, length, chemotaxis, motility and drift behind
-// runtime flags; the plain variant prices contact, volume and surface only.
-// MOORE2D fixes the eight planar Moore offsets at compile time.
 template <bool D3, bool FULL, bool MOORE2D>
 __global__ __launch_bounds__(512) void sweep_kernel(SweepArgs a) {
-    extern __shared__ uint32_t lds[];
-    const int tid = threadIdx.x;
-    const int nthreads = blockDim.x;
-    const DeviceTables *tab = a.tables;
-
-    // ---- block -> tile, keeping each XCD on a contiguous run of tiles so
-    // shared halo rows stay in one L2.
-    int b = blockIdx.x;
-    if (a.xcd_chunk > 0) b = (b & 7) * a.xcd_chunk + (b >> 3);
-    const int tile_x = b % a.tiles_x;
-    const int tile_y = (b / a.tiles_x) % a.tiles_y;
-    const int tile_z = D3 ? b / (a.tiles_x * a.tiles_y) : 0;
-    const int TX = 1 << a.tx_log2, TY = 1 << a.ty_log2, TZ = D3 ? 1 << a.tz_log2 : 1;
-    const int halo = a.halo;
-    const int LX = 2 * TX + 2 * halo, LY = 2 * TY + 2 * halo, LZ = D3 ? 2 * TZ + 2 * halo : 1;
-    const int x0 = tile_x * 2 * TX, y0 = tile_y * 2 * TY, z0 = D3 ? tile_z * 2 * TZ : 0;
-
-    // ---- LDS: contact table, per-type volume/surface, acceptance table,
-    // neighbour offsets, then the label tile (and the activity tile).
-    const int T = (int)a.n_types;
-    float *sJ = reinterpret_cast<float *>(lds);
-    float *sP = sJ + T * T;                       // 4 per type
-    uint32_t *sThr = reinterpret_cast<uint32_t *>(sP + 4 * T);
-    int *sOff = reinterpret_cast<int *>(sThr + a.n_thr); // [0,32) LDS deltas, [32,64) packed xyz
-    uint32_t *sLab = reinterpret_cast<uint32_t *>(sOff + 64);
-    float *sAct = reinterpret_cast<float *>(sLab + LX * LY * LZ);
-
-    for (int i = tid; i < T * T; i += nthreads) sJ[i] = tab->J[i / T][i % T];
-    for (int i = tid; i < T; i += nthreads) {
-        sP[4 * i + 0] = tab->lam_v[i];
-        sP[4 * i + 1] = tab->tgt_v[i];
-        sP[4 * i + 2] = tab->lam_s[i];
-        sP[4 * i + 3] = tab->tgt_s[i];
-    }
-    for (int i = tid; i < (int)a.n_thr; i += nthreads) sThr[i] = tab->thr[i];
-    if (tid < a.n_off) {
-        sOff[tid] = a.off[tid][0] + a.off[tid][1] * LX + a.off[tid][2] * LX * LY;
-        sOff[32 + tid] = (a.off[tid][0] + 1) | ((a.off[tid][1] + 1) << 2) | ((a.off[tid][2] + 1) << 4);
-    }
-
-    // ---- fold this block's share of the per-label snapshot for the next
-    // pass, and clear the delta buffer two passes ahead.
-    {
-        const uint64_t words = (uint64_t)(*a.n_labels) * (uint64_t)a.cell_words;
-        uint64_t lo = (uint64_t)blockIdx.x * a.fold_per_block;
-        uint64_t hi = lo + a.fold_per_block;
-        if (hi > words) hi = words;
-        for (uint64_t i = lo + tid; i < hi; i += nthreads) {
-            a.a_next[i] = a.a_cur[i] + a.e_prev[i];
-            a.g_zero[i] = 0;
-        }
-    }
-
-    // ---- stage labels (and activity) with whole-row loads.
-    const uint32_t *src_lab = (FULL && (a.flags & kMotility)) ? a.labels_prev : a.labels;
-    const int rows = LY * LZ;
-    const int wave = tid >> 6, lane = tid & 63, nwaves = nthreads >> 6;
-    for (int r = wave; r < rows; r += nwaves) {
-        int ly = r % LY, lz = r / LY;
-        int gy = wrap_coord(y0 - halo + ly, a.h);
-        int gz = D3 ? wrap_coord(z0 - halo + lz, a.d) : 0;
-        size_t row = ((size_t)gz * a.h + gy) * (size_t)a.w;
-        uint32_t *dst = sLab + r * LX;
-        for (int c = lane; c < LX; c += 64) {
-            int gx = wrap_coord(x0 - halo + c, a.w);
-            dst[c] = src_lab[row + gx];
-            if (FULL && (a.flags & kMotility)) sAct[r * LX + c] = a.act_prev[row + gx];
-        }
-    }
-    __syncthreads();
-
-    // ---- this lane's target.
-    const int i = tid & (TX - 1);
-    const int j = (tid >> a.tx_log2) & (TY - 1);
-    const int k = D3 ? (tid >> (a.tx_log2 + a.ty_log2)) : 0;
-    const int gx = x0 + 2 * i + a.cx, gy = y0 + 2 * j + a.cy, gz = D3 ? z0 + 2 * k + a.cz : 0;
-    bool accepted = false;
-    int tl_keep = 0;
-    uint32_t nw_keep = 0;
-    float act_keep = -1.0f;
-    const bool active = tid < TX * TY * TZ && gx < a.w && gy < a.h && (!D3 || gz < a.d);
-    if (active) {
-        const int lx = halo + 2 * i + a.cx, ly = halo + 2 * j + a.cy, lz = D3 ? halo + 2 * k + a.cz : 0;
-        const int tl = (lz * LY + ly) * LX + lx;
-        const uint32_t site = (uint32_t)(((size_t)gz * a.h + gy) * (size_t)a.w + gx);
-        const uint32_t h = site_draw(a.key, a.key2, site);
-        int pick;
-        uint32_t r24;
-        int pdx, pdy, pdz;
-        if (MOORE2D) {
-            pick = (int)(h & 7u);
-            r24 = h >> 8;
-            int q = pick + (pick >= 4 ? 1 : 0);
-            pdy = q / 3 - 1;
-            pdx = q - 3 * (pdy + 1) - 1;
-            pdz = 0;
-        } else {
-            const int n = a.n_off;
-            if ((n & (n - 1)) == 0) {
-                pick = (int)(h & (uint32_t)(n - 1));
-                r24 = h >> 8;
-            } else {
-                pick = (int)(((uint64_t)h * (uint64_t)n) >> 32);
-                r24 = second_uniform(h);
-            }
-            const int packed = sOff[32 + pick];
-            pdx = (packed & 3) - 1;
-            pdy = ((packed >> 2) & 3) - 1;
-            pdz = ((packed >> 4) & 3) - 1;
-        }
-        const int sl = tl + pdx + pdy * LX + pdz * LX * LY;
-        const uint32_t old = sLab[tl];
-        const uint32_t nw = sLab[sl];
-        const uint32_t oid = label_id(old), nid = label_id(nw);
-        const int cw = a.cell_words;
-        int64_t pv_old = 0, pv_new = 0;
-        if (oid != 0) pv_old = a.a_cur[(size_t)oid * cw] + a.e_prev[(size_t)oid * cw];
-        if (nid != 0) pv_new = a.a_cur[(size_t)nid * cw] + a.e_prev[(size_t)nid * cw];
-        bool go = old != nw && !(oid != 0 && unpack_vol(pv_old) <= 1);
-
-        if (FULL && go && oid != 0 && (a.flags & kConnected) && tab->connected[label_type(old)]) {
-            uint32_t present = 0u;
-            const int rn = D3 ? 26 : 8;
-            for (int q = 0; q < rn; ++q) {
-                int dx, dy, dz;
-                ring_offset(q, D3, &dx, &dy, &dz);
-                if (sLab[tl + dx + dy * LX + dz * LX * LY] == old) present |= 1u << q;
-            }
-            go = one_piece(present, D3, tab);
-        }
-
-        if (go) {
-            const uint32_t eto = label_type(old), etn = label_type(nw);
-            float cn = 0.0f, co = 0.0f;
-            int like_old = 0, like_new = 0;
-            const float *rowN = sJ + etn * T;
-            const float *rowO = sJ + eto * T;
-            if (MOORE2D) {
-#pragma unroll
-                for (int q = 0; q < 8; ++q) {
-                    int qq = q + (q >= 4 ? 1 : 0);
-                    int dy = qq / 3 - 1, dx = qq % 3 - 1;
-                    uint32_t ln = sLab[tl + dx + dy * LX];
-                    uint32_t tn = label_type(ln);
-                    if (ln != nw) cn = cn + rowN[tn]; else ++like_new;
-                    if (ln != old) co = co + rowO[tn]; else ++like_old;
-                }
-            } else {
-                for (int q = 0; q < a.n_off; ++q) {
-                    uint32_t ln = sLab[tl + sOff[q]];
-                    uint32_t tn = label_type(ln);
-                    if (ln != nw) cn = cn + rowN[tn]; else ++like_new;
-                    if (ln != old) co = co + rowO[tn]; else ++like_old;
-                }
-            }
-            const int bonds = MOORE2D ? 8 : a.n_off;
-            float dh = cn - co;
-            const int ds_old = 2 * like_old - bonds;
-            const int ds_new = bonds - 2 * like_new;
-            if (oid != 0) {
-                float lv = sP[4 * eto + 0];
-                if (lv != 0.0f) {
-                    float v = (float)unpack_vol(pv_old);
-                    dh = dh + lv * (1.0f - 2.0f * (v - sP[4 * eto + 1]));
-                }
-            }
-            if (nid != 0) {
-                float lv = sP[4 * etn + 0];
-                if (lv != 0.0f) {
-                    float v = (float)unpack_vol(pv_new);
-                    dh = dh + lv * (1.0f + 2.0f * (v - sP[4 * etn + 1]));
-                }
-            }
-            if (oid != 0 && ds_old != 0) {
-                float ls = sP[4 * eto + 2];
-                if (ls != 0.0f) {
-                    float s = (float)unpack_surf(pv_old);
-                    float dsf = (float)ds_old;
-                    dh = dh + ls * (dsf * (2.0f * (s - sP[4 * eto + 3]) + dsf));
-                }
-            }
-            if (nid != 0 && ds_new != 0) {
-                float ls = sP[4 * etn + 2];
-                if (ls != 0.0f) {
-                    float s = (float)unpack_surf(pv_new);
-                    float dsf = (float)ds_new;
-                    dh = dh + ls * (dsf * (2.0f * (s - sP[4 * etn + 3]) + dsf));
-                }
-            }
-            int sx = 0, sy = 0, sz = 0;
-            if (FULL) {
-                sx = wrap_coord(gx + pdx, a.w);
-                sy = wrap_coord(gy + pdy, a.h);
-                sz = D3 ? wrap_coord(gz + pdz, a.d) : 0;
-            }
-            if (FULL && (a.flags & kLength)) {
-                const bool three = D3;
-                for (int side = 0; side < 2; ++side) {
-                    const uint32_t id = side == 0 ? oid : nid;
-                    const uint32_t ty = side == 0 ? eto : etn;
-                    if (id == 0 || tab->lam_len[ty] == 0.0) continue;
-                    const int sign = side == 0 ? -1 : 1;
-                    const size_t base = (size_t)id * cw;
-                    int64_t m[9], after[9];
-                    for (int q = 0; q < 9; ++q) m[q] = a.a_cur[base + 1 + q] + a.e_prev[base + 1 + q];
-                    const int64_t n = unpack_vol(side == 0 ? pv_old : pv_new);
-                    const int64_t ux = fold_int(gx - a.anchor[3 * id], a.w);
-                    const int64_t uy = fold_int(gy - a.anchor[3 * id + 1], a.h);
-                    const int64_t uz = fold_int(gz - a.anchor[3 * id + 2], a.d);
-                    after[0] = m[0] + sign * ux;
-                    after[1] = m[1] + sign * uy;
-                    after[2] = m[2] + sign * uz;
-                    after[3] = m[3] + sign * ux * ux;
-                    after[4] = m[4] + sign * uy * uy;
-                    after[5] = m[5] + sign * uz * uz;
-                    after[6] = m[6] + sign * ux * uy;
-                    after[7] = m[7] + sign * ux * uz;
-                    after[8] = m[8] + sign * uy * uz;
-                    double before = length_of(m, n, three) - tab->tgt_len[ty];
-                    double now = length_of(after, n + sign, three) - tab->tgt_len[ty];
-                    dh = dh + (float)(tab->lam_len[ty] * (now * now - before * before));
-                }
-            }
-            if (FULL && (a.flags & kChemotaxis) && nid != 0) {
-                const size_t ssite = ((size_t)sz * a.h + sy) * (size_t)a.w + sx;
-                for (int s = 0; s < a.n_species; ++s) {
-                    float chi = tab->chi[etn][s];
-                    if (chi == 0.0f) continue;
-                    dh = dh - chi * (a.fields[s][site] - a.fields[s][ssite]);
-                }
-            }
-            if (FULL && (a.flags & kMotility) && nid != 0 && tab->coef_act[etn] != 0.0f &&
-                tab->max_act[etn] > 0.0f) {
-                float gm[2];
-                for (int side = 0; side < 2; ++side) {
-                    const int at = side == 0 ? sl : tl;
-                    const uint32_t lab = side == 0 ? nw : old;
-                    float here = sAct[at];
-                    float res = 0.0f;
-                    if (here > 0.0f) {
-                        float lsum = log2_f32(here);
-                        float count = 1.0f;
-                        bool zero = false;
-                        for (int q = 0; q < a.n_off; ++q) {
-                            int nb = at + sOff[q];
-                            if (sLab[nb] != lab) continue;
-                            float v = sAct[nb];
-                            if (!(v > 0.0f)) {
-                                zero = true;
-                                break;
-                            }
-                            lsum = lsum + log2_f32(v);
-                            count = count + 1.0f;
-                        }
-                        res = zero ? 0.0f : exp2_f32(lsum / count);
-                    }
-                    gm[side] = res;
-                }
-                dh = dh - tab->coef_act[etn] * (gm[0] - gm[1]);
-            }
-            if (FULL && (a.flags & kDrift) && nid != 0) {
-                float wk = tab->ext[etn][0] * (float)(-pdx) + tab->ext[etn][1] * (float)(-pdy) +
-                           tab->ext[etn][2] * (float)(-pdz);
-                dh = dh - wk;
-            }
-
-            bool acc;
-            if (dh <= 0.0f) {
-                acc = true;
-            } else if (a.flags & kIntegral) {
-                acc = dh < (float)a.n_thr && r24 < sThr[(int)dh];
-            } else {
-                float p = exp2_f32(-dh * tab->accept_c);
-                acc = (float)r24 < p * 16777216.0f;
-            }
-
-            if (acc) {
-                accepted = true;
-                if (!(FULL && (a.flags & kMotility))) a.labels[site] = nw;
-                if (oid != 0)
-                    atomicAdd(reinterpret_cast<unsigned long long *>(a.f_acc + (size_t)oid * cw),
-                              (unsigned long long)pack_vs(-1, ds_old));
-                if (nid != 0)
-                    atomicAdd(reinterpret_cast<unsigned long long *>(a.f_acc + (size_t)nid * cw),
-                              (unsigned long long)pack_vs(1, ds_new));
-                if (FULL && (a.flags & kMoments)) {
-                    for (int side = 0; side < 2; ++side) {
-                        const uint32_t id = side == 0 ? oid : nid;
-                        if (id == 0) continue;
-                        const int64_t sg = side == 0 ? -1 : 1;
-                        const int64_t ux = fold_int(gx - a.anchor[3 * id], a.w);
-                        const int64_t uy = fold_int(gy - a.anchor[3 * id + 1], a.h);
-                        const int64_t uz = fold_int(gz - a.anchor[3 * id + 2], a.d);
-                        const int64_t dm[9] = {sg * ux,      sg * uy,      sg * uz,
-                                               sg * ux * ux, sg * uy * uy, sg * uz * uz,
-                                               sg * ux * uy, sg * ux * uz, sg * uy * uz};
-                        unsigned long long *dst =
-                            reinterpret_cast<unsigned long long *>(a.f_acc + (size_t)id * cw + 1);
-                        for (int q = 0; q < 9; ++q)
-                            if (dm[q] != 0) atomicAdd(dst + q, (unsigned long long)dm[q]);
-                    }
-                }
-                if (FULL && (a.flags & kMotility)) {
-                    // written through the tile once every lane has priced
-                    tl_keep = tl;
-                    nw_keep = nw;
-                    if (nid != 0 && tab->max_act[etn] > 0.0f) act_keep = tab->max_act[etn];
-                }
-            }
-        }
-    }
-
-    // ---- acceptance count: one atomic per wavefront into a spread slot.
-    const unsigned long long ball = __ballot(accepted);
-    if (lane == 0 && ball)
-        atomicAdd(a.accepted + (blockIdx.x & (kAcceptSlots - 1)),
-                  (unsigned long long)__builtin_popcountll(ball));
-
-    // ---- with motility the pass reads a pass-start copy and writes the
-    // whole tile interior into the other buffer, so that buffer is complete.
-    if (FULL && (a.flags & kMotility)) {
-        __syncthreads();
-        if (accepted) {
-            sLab[tl_keep] = nw_keep;
-            if (act_keep >= 0.0f) sAct[tl_keep] = act_keep;
-        }
-        __syncthreads();
-        const int IX = 2 * TX, IY = 2 * TY, IZ = D3 ? 2 * TZ : 1;
-        for (int r = wave; r < IY * IZ; r += nwaves) {
-            int iy = r % IY, iz = r / IY;
-            int gyy = y0 + iy, gzz = z0 + iz;
-            if (gyy >= a.h || (D3 && gzz >= a.d)) continue;
-            size_t row = ((size_t)gzz * a.h + gyy) * (size_t)a.w;
-            int lrow = ((D3 ? iz + halo : 0) * LY + iy + halo) * LX + halo;
-            for (int c = lane; c < IX; c += 64) {
-                if (x0 + c >= a.w) continue;
-                a.labels[row + x0 + c] = sLab[lrow + c];
-                a.act[row + x0 + c] = sAct[lrow + c];
-            }
-        }
-    }
+    sweep_body<D3, FULL, MOORE2D>(a, blockIdx.x);
 }
 
 // Bytes of dynamic LDS a launch with these tile sizes needs.

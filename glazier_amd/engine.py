@@ -69,6 +69,11 @@ def library() -> ctypes.CDLL:
     lib.ghip_cell_state.argtypes = [sim, P(ctypes.c_int64), ctypes.c_size_t, P(ctypes.c_uint32)]
     lib.ghip_set_timing.argtypes = [sim, ctypes.c_int]
     lib.ghip_kernel_time.argtypes = [sim, ctypes.c_char_p, P(ctypes.c_double), P(ctypes.c_uint64)]
+    lib.ghip_batch_create.argtypes = [P(ctypes.c_void_p), ctypes.c_size_t, P(ctypes.c_void_p)]
+    lib.ghip_batch_destroy.argtypes = [ctypes.c_void_p]
+    lib.ghip_batch_step.argtypes = [ctypes.c_void_p, ctypes.c_uint64]
+    lib.ghip_batch_set_tiles.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    lib.ghip_batch_describe.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_size_t]
     lib.ghip_last_error.restype = ctypes.c_char_p
     _lib = lib
     return lib
@@ -116,7 +121,7 @@ class HipSimulation:
 
     def close(self) -> None:
         if getattr(self, "_h", None) and self._h.value:
-            library().ghip_destroy(self._h)
+            _check(library().ghip_destroy(self._h))
             self._h = ctypes.c_void_p()
 
     def __del__(self):
@@ -208,3 +213,54 @@ class HipSimulation:
         ms, n = ctypes.c_double(), ctypes.c_uint64()
         _check(library().ghip_kernel_time(self._h, family.encode(), ctypes.byref(ms), ctypes.byref(n)))
         return float(ms.value), int(n.value)
+
+
+class HipBatch:
+    """Several HipSimulation lattices stepped together on one GPU.
+
+    Each stage of a Monte Carlo step is one launch over every lattice, and
+    after the first steps (which time the candidate tile shapes on real
+    steps) the whole step is a replayed hipGraph. Every lattice computes
+    exactly what its own `step` would; read it through its HipSimulation
+    between batch steps. The batch holds its lattices and is closed before
+    them.
+    """
+
+    def __init__(self, sims: Sequence[HipSimulation]):
+        lib = library()
+        self.sims = list(sims)
+        handles = (ctypes.c_void_p * len(self.sims))(*[s._h.value for s in self.sims])
+        handle = ctypes.c_void_p()
+        _check(lib.ghip_batch_create(handles, len(self.sims), ctypes.byref(handle)))
+        self._h = handle
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) and self._h.value:
+            _check(library().ghip_batch_destroy(self._h))
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def step(self, steps: int = 1) -> None:
+        _check(library().ghip_batch_step(self._h, int(steps)))
+
+    def set_tiles(self, candidate: int) -> None:
+        """Fix every group to tile candidate 0..4, or -1 to time them again."""
+        _check(library().ghip_batch_set_tiles(self._h, int(candidate)))
+
+    def describe(self) -> dict:
+        import json
+
+        buf = ctypes.create_string_buffer(1 << 16)
+        _check(library().ghip_batch_describe(self._h, buf, len(buf)))
+        return json.loads(buf.value.decode())

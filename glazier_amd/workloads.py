@@ -12,6 +12,10 @@ Each builds a Model, a starting lattice and a type per label:
       0.05 decay 0.02) on 4 substeps, secretion 1.0 / 0.5 by infected cells,
       uptake 0.05 by epithelial cells, chemotaxis 5 toward interferon, length
       lambda_L 1 L_t 10, infected death 0.002 per step, division at 128.
+  ensemble(seed)  64 planar lattices of the infection physics with sides
+      drawn by seed from {64, 128, 256, 512}, plus 16 periodic 64^3 volumes
+      of 8^3 cubes (26-neighbour), each with its own seed and replica index
+      and with J ~ U[5, 15], lambda_V ~ U[1, 4], T ~ U[5, 20] drawn by seed.
 """
 
 from __future__ import annotations
@@ -57,3 +61,53 @@ def infection_sheet(side: int = 512, seed: int = 7, infected_fraction: float = 0
     kinds[infected] = 2
     types = default_cell_types(cells + 1, kinds)
     return model, labels, types
+
+
+def infection_model(width: int, height: int, depth: int, seed: int, contact: float = 10.0,
+                    lambda_volume: float = 2.0, temperature: float = 10.0,
+                    target_volume: float = 64.0) -> Model:
+    """The infection physics on any lattice; the division volume is twice the
+    target volume (128 on the planar sheet)."""
+    epi = CellType(target_volume=target_volume, lambda_volume=lambda_volume, target_length=10.0,
+                   lambda_length=1.0)
+    inf = CellType(target_volume=target_volume, lambda_volume=lambda_volume, target_length=10.0,
+                   lambda_length=1.0, death_rate=0.002, division_volume=2.0 * target_volume)
+    j = float(contact)
+    return Model(
+        width=width, height=height, depth=depth,
+        contact=[0.0, j, j, j, j, j, j, j, j],
+        types=[CellType(), epi, inf], temperature=float(temperature),
+        neighbour_order=2 if depth == 1 else 3, seed=seed,
+        species=[Species("virus", 0.1, 0.01, 0.0, substeps=4),
+                 Species("interferon", 0.05, 0.02, 0.0, substeps=4)],
+        secretion=[[0.0, 0.0], [0.0, 0.0], [1.0, 0.5]],
+        uptake=[[0.0, 0.0], [0.05, 0.05], [0.0, 0.0]],
+        chemotaxis=[[0.0, 0.0], [0.0, 5.0], [0.0, 5.0]],
+    )
+
+
+def ensemble(seed: int = 3, planar: int = 64, volumes: int = 16,
+             sides=(64, 128, 256, 512), volume_side: int = 64):
+    """[(model, labels, types)] of the batched ensemble: `planar` infection
+    sheets of sides drawn from `sides`, then `volumes` periodic cubes of
+    `volume_side`^3 tiled by 8^3 cells. Replica r has seed 1000 + r and
+    replica index r; parameters are drawn from one generator seeded by `seed`."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for r in range(planar + volumes):
+        three_d = r >= planar
+        side = volume_side if three_d else int(rng.choice(sides))
+        j = float(rng.uniform(5.0, 15.0))
+        lam = float(rng.uniform(1.0, 4.0))
+        temp = float(rng.uniform(5.0, 20.0))
+        depth = side if three_d else 1
+        m = infection_model(side, side, depth, 1000 + r, contact=j, lambda_volume=lam,
+                            temperature=temp, target_volume=512.0 if three_d else 64.0)
+        m.replica = r
+        n = side // 8
+        labels = tiled_labels(m, 8, n, n, n if three_d else 1)
+        cells = n ** (3 if three_d else 2)
+        kinds = np.ones(cells, dtype=np.uint8)
+        kinds[rng.choice(cells, size=max(1, int(round(0.05 * cells))), replace=False)] = 2
+        out.append((m, labels, default_cell_types(cells + 1, kinds)))
+    return out

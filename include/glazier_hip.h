@@ -162,6 +162,36 @@ ghip_status ghip_set_timing(ghip_sim *sim, int on);
 ghip_status ghip_kernel_time(const ghip_sim *sim, const char *family, double *total_ms,
                              uint64_t *launches);
 
+/* ---- Batches. No counterpart in the reference, which steps one lattice per
+ * process (glazier/src/main.rs:137-160, examples/bench.rs:40-60); an
+ * ensemble there is a loop over processes or over GpuSimulation handles.
+ *
+ * A batch steps several lattices together: each stage of a Monte Carlo step
+ * (colour passes, field launches, per-label work) is one launch over every
+ * lattice taking part, and after the first steps the whole step is one
+ * replayed hipGraph. The lattices keep their own seeds, replica indices,
+ * models and sizes (2D and 3D may be mixed) and compute exactly what
+ * ghip_step computes for each alone. The handles stay valid: read them with
+ * the functions above between batch steps; ghip_step on one of them is
+ * allowed and the batch picks up where it left it. Destroy the batch before
+ * its lattices (ghip_destroy refuses a lattice that is in a batch). */
+typedef struct ghip_batch ghip_batch;
+
+ghip_status ghip_batch_create(ghip_sim *const *sims, size_t n, ghip_batch **out);
+ghip_status ghip_batch_destroy(ghip_batch *batch);
+
+/* Runs `steps` Monte Carlo steps of every lattice in the batch. The first
+ * six steps of a new batch time each candidate tile shape of the colour
+ * passes on real steps and keep the fastest per kernel variant. */
+ghip_status ghip_batch_step(ghip_batch *batch, uint64_t steps);
+
+/* Fixes every group to one tile candidate (0..4) and skips the timing
+ * steps, or with -1 starts the timing over. */
+ghip_status ghip_batch_set_tiles(ghip_batch *batch, int candidate);
+
+/* A JSON description of the groups, their chosen tiles and timings. */
+ghip_status ghip_batch_describe(const ghip_batch *batch, char *out, size_t n);
+
 /* Message of the last failure on this thread; empty when none. */
 const char *ghip_last_error(void);
 
