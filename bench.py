@@ -10,6 +10,12 @@ Inputs are resident in HBM before the timed region starts.
 Prints one JSON line with site-attempts/s, MCS/s, the roofline of the sweep
 kernel (algorithmic bytes / measured launch time against 8 TB/s) and a CPU
 baseline (the serial oracle on this host, one thread, on a bounded sample).
+
+The same line carries "ensemble": the batched-versus-looped comparison on
+the uneven ensemble (64 infection sheets of 64^2..512^2 plus 16 volumes of
+64^3, parameters drawn by seed 3), each lattice run for --ensemble-mcs steps
+once as one batch and once lattice by lattice; replicas/s is lattices
+finished per second of wall time, setup excluded.
 """
 
 from __future__ import annotations
@@ -61,6 +67,70 @@ def cpu_baseline(side: int, seconds: float):
                       f"({cpu_model()})"}
 
 
+def ensemble_bench(mcs: int):
+    """Batched against looped on the uneven ensemble: the same lattices from
+    the same starting state, `mcs` steps each, timed from the first launch to
+    the last completion (device allocation and upload excluded)."""
+    import torch
+
+    from glazier_amd import HipBatch, HipSimulation
+    from glazier_amd.workloads import ensemble
+
+    members = ensemble(seed=3)
+    sites = sum(int(l.size) for _, l, _ in members)
+    # load every kernel variant once, alone and batched, outside the timing
+    warm = [HipSimulation(*members[i]) for i in (0, len(members) - 1)]
+    for w in warm:
+        w.step(1)
+    wb = HipBatch(warm)
+    wb.set_tiles(0)
+    wb.step(1)
+    wb.close()
+    for w in warm:
+        w.close()
+
+    sims = [HipSimulation(m, l, t) for m, l, t in members]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for s in sims:
+        s.step(mcs)
+    torch.cuda.synchronize()
+    looped = time.perf_counter() - t0
+    looped_labels = [s.labels() for s in sims]
+    for s in sims:
+        s.close()
+
+    sims = [HipSimulation(m, l, t) for m, l, t in members]
+    batch = HipBatch(sims)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    batch.step(mcs)
+    torch.cuda.synchronize()
+    batched = time.perf_counter() - t0
+    same = all((s.labels() == want).all() for s, want in zip(sims, looped_labels))
+    # the captured step alone, after tuning
+    t0 = time.perf_counter()
+    batch.step(10)
+    torch.cuda.synchronize()
+    graph_step = (time.perf_counter() - t0) / 10
+    info = batch.describe()
+    batch.close()
+    for s in sims:
+        s.close()
+    n = len(members)
+    return {
+        "workload": "64 infection sheets (sides by seed from 64..512) + 16 x 64^3 volumes, "
+                    "J~U[5,15], lambda_V~U[1,4], T~U[5,20], seed 3",
+        "lattices": n, "sites": sites, "mcs_each": mcs,
+        "batched_replicas_per_s": n / batched, "looped_replicas_per_s": n / looped,
+        "speedup": looped / batched, "batched_s": batched, "looped_s": looped,
+        "batched_site_attempts_per_s": sites * mcs / batched,
+        "graph_step_ms": graph_step * 1e3, "lattices_identical": bool(same),
+        "tiles": [{"dims": g["dims"], "lattices": g["lattices"], "tile": g["tile"],
+                   "threads": g["threads"]} for g in info["groups"]],
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -69,6 +139,8 @@ def main() -> int:
     ap.add_argument("--side", type=int, default=4096)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--ensemble-mcs", type=int, default=100)
+    ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy a MIB-mebibyte buffer 5 times with torch, so a counter "
                          "run can scale FETCH_SIZE/WRITE_SIZE by a copy of known size")
@@ -151,12 +223,14 @@ def main() -> int:
         "hbm_frac_algorithmic": attempts * (16.0 + 4.0 * a) / elapsed / (HBM_PEAK_GBS * 1e9),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "sweep_kernel (one colour pass)",
+                     "kernel": "sweep_plain2d_kernel (one colour pass)",
                      "avg_launch_us": avg_s * 1e6, "algorithmic_bytes_per_launch": bytes_per_launch},
         "cpu_baseline": None,
     }
     for s in sims:
         s.close()
+    if not args.no_ensemble:
+        line["ensemble"] = ensemble_bench(args.ensemble_mcs)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(side, args.cpu_seconds)
     print(json.dumps(line))

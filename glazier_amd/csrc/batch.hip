@@ -40,17 +40,19 @@ __device__ __forceinline__ int flip_index(const BatchReplica &R, int base, uint6
 // ---------------------------------------------------------------- sweep
 
 // One thread per replica: the arguments of pass `phase` of its current step,
-// as the single-lattice driver builds them (engine.cpp, run_sweep_pass).
+// as the single-lattice driver builds them (engine.cpp, run_sweep_pass). The
+// record already holds the replica's static arguments (uploaded with the
+// replica records); only the per-pass ones are written.
 __global__ void batch_prepare_sweep_kernel(BatchArgs b, int phase) {
     const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= b.n_reps) return;
     const BatchReplica &R = b.reps[r];
-    SweepArgs a = R.sweep;
+    SweepArgs &a = b.cur_sweep[r];
     if (phase >= R.colours) {
         a.n_off = 0;  // no pass in this phase
-        b.cur_sweep[r] = a;
         return;
     }
+    a.n_off = R.sweep.n_off;
     const uint64_t m = b.mcs[r];
     int order[8];
     colour_order(R.seed, R.replica, m, R.colours, order);
@@ -75,7 +77,6 @@ __global__ void batch_prepare_sweep_kernel(BatchArgs b, int phase) {
     a.cz = (colour >> 2) & 1;
     a.key = pass_key(R.seed, R.replica, m, (uint32_t)colour);
     a.key2 = second_key(a.key);
-    b.cur_sweep[r] = a;
 }
 
 template <bool D3, bool FULL, bool MOORE2D>
@@ -134,19 +135,19 @@ __global__ __launch_bounds__(256) void batch_field_kernel(BatchArgs b, WorkList 
 // ---------------------------------------------------------------- cells
 
 // One thread per replica: this step's per-label arguments (engine.cpp,
-// run_cells) and the activity buffer the decay steps down.
+// run_cells) and the activity buffer the decay steps down. As for the
+// sweep, the static arguments are already in the record.
 __global__ void batch_prepare_cells_kernel(BatchArgs b) {
     const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= b.n_reps) return;
     const BatchReplica &R = b.reps[r];
-    CellArgs c = R.cell;
+    CellArgs &c = b.cur_cell[r];
     const uint64_t m = b.mcs[r];
     const uint64_t p_end = passes_before(R, m + 1);
     c.state = R.A[p_end % 2];
     c.labels = R.labels[flip_index(R, R.lab0, p_end)];
     c.key = pass_key(R.seed, R.replica, m, 0xDEADCE11U);
     c.key2 = second_key(c.key);
-    b.cur_cell[r] = c;
     b.cur_act[r] = R.act[flip_index(R, R.act0, p_end)];
 }
 

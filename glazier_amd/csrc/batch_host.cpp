@@ -335,6 +335,16 @@ ghip_status rebuild(ghip_batch *b) {
     for (auto &f : b->fields) f.w = materialise(b, f.ref);
     for (auto &c : b->cells) c.w = materialise(b, c.ref);
     HIP_TRY(hipMemcpy(b->d_reps, b->reps.data(), R * sizeof(BatchReplica), hipMemcpyHostToDevice));
+    // the per-launch records start as the static arguments; the prepare
+    // kernels write only what changes from pass to pass
+    std::vector<SweepArgs> sweep(R);
+    std::vector<CellArgs> cell(R);
+    for (size_t r = 0; r < R; ++r) {
+        sweep[r] = b->reps[r].sweep;
+        cell[r] = b->reps[r].cell;
+    }
+    HIP_TRY(hipMemcpy(b->d_sweep, sweep.data(), R * sizeof(SweepArgs), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(b->d_cell, cell.data(), R * sizeof(CellArgs), hipMemcpyHostToDevice));
     std::vector<uint64_t> mcs(R);
     for (size_t r = 0; r < R; ++r) mcs[r] = b->sims[r]->mcs;
     HIP_TRY(hipMemcpy(b->d_mcs, mcs.data(), R * 8, hipMemcpyHostToDevice));

@@ -22,12 +22,20 @@
 namespace ghip {
 
 namespace {
+// Coordinates of site i; lattices hold fewer than 2^32 sites, so 32-bit
+// division suffices.
 __device__ __forceinline__ void coords(size_t i, int w, int h, int *x, int *y, int *z) {
-    size_t plane = (size_t)w * h;
-    *z = (int)(i / plane);
-    size_t r = i % plane;
-    *y = (int)(r / w);
-    *x = (int)(r % w);
+    const uint32_t s = (uint32_t)i, plane = (uint32_t)w * (uint32_t)h;
+    const uint32_t zz = s / plane, r = s - zz * plane;
+    const uint32_t yy = r / (uint32_t)w;
+    *z = (int)zz;
+    *y = (int)yy;
+    *x = (int)(r - yy * (uint32_t)w);
+}
+
+// v taken onto [0, span) when it lies at most one span outside.
+__device__ __forceinline__ int wrap1(int v, int span) {
+    return v < 0 ? v + span : (v >= span ? v - span : v);
 }
 }  // namespace
 
@@ -52,13 +60,13 @@ __device__ __forceinline__ void consolidate_body(int64_t *a, int64_t *e, const u
 }
 
 // Lowest site index of every label.
-__device__ __forceinline__ void first_site_body(CellArgs a, size_t n_sites, const int lb, const int gb) {
+__device__ __forceinline__ void first_site_body(const CellArgs &a, size_t n_sites, const int lb, const int gb) {
     size_t i = (size_t)lb * blockDim.x + threadIdx.x;
     if (i >= n_sites) return;
     atomicMin(a.scratch_u32 + label_id(a.labels[i]), (uint32_t)i);
 }
 
-__device__ __forceinline__ void anchor_from_first_body(CellArgs a, const int lb, const int gb) {
+__device__ __forceinline__ void anchor_from_first_body(const CellArgs &a, const int lb, const int gb) {
     uint32_t l = lb * blockDim.x + threadIdx.x;
     if (l >= *a.n_labels) return;
     uint32_t f = a.scratch_u32[l];
@@ -77,7 +85,7 @@ __device__ __forceinline__ void anchor_from_first_body(CellArgs a, const int lb,
 // flags[gate] is set, so a whole step can be enqueued without host checks.
 constexpr int kSlots = 256;
 
-__device__ __forceinline__ void recount_body(CellArgs a, size_t n_sites, int sites_per_block,
+__device__ __forceinline__ void recount_body(const CellArgs &a, size_t n_sites, int sites_per_block,
                                                       int moments_only, int gate, const int lb, const int gb) {
     if (gate >= 0 && a.flags[gate] == 0) return;
     __shared__ uint32_t keys[kSlots];
@@ -95,13 +103,15 @@ __device__ __forceinline__ void recount_body(CellArgs a, size_t n_sites, int sit
         int x, y, z;
         coords(i, a.w, a.h, &x, &y, &z);
         int64_t c[10];
+#pragma unroll
         for (int q = 0; q < 10; ++q) c[q] = 0;
         if (!moments_only) {
+            // neighbour offsets are within one site on every axis
             int bonds = 0;
             for (int k = 0; k < a.n_off; ++k) {
-                size_t nb = ((size_t)wrapc(z + a.off[k][2], a.d) * a.h + wrapc(y + a.off[k][1], a.h)) *
-                                (size_t)a.w +
-                            wrapc(x + a.off[k][0], a.w);
+                const int nx = wrap1(x + a.off[k][0], a.w), ny = wrap1(y + a.off[k][1], a.h),
+                          nz = wrap1(z + a.off[k][2], a.d);
+                const size_t nb = ((size_t)nz * a.h + ny) * (size_t)a.w + nx;
                 if (nb != i && a.labels[nb] != word) ++bonds;
             }
             c[0] = pack_vs(1, bonds);
@@ -130,8 +140,9 @@ __device__ __forceinline__ void recount_body(CellArgs a, size_t n_sites, int sit
             }
             slot = (slot + 1) & (kSlots - 1);
         }
-        for (int q = 0; q < cw; ++q) {
-            if (c[q] == 0) continue;
+#pragma unroll
+        for (int q = 0; q < 10; ++q) {
+            if (q >= cw || c[q] == 0) continue;
             if (found >= 0)
                 atomicAdd(&vals[found * cw + q], (unsigned long long)c[q]);
             else
@@ -148,7 +159,7 @@ __device__ __forceinline__ void recount_body(CellArgs a, size_t n_sites, int sit
 }
 
 // Zero the moment words (1..9) of every label, keeping volume and surface.
-__device__ __forceinline__ void zero_moments_body(CellArgs a, const int lb, const int gb) {
+__device__ __forceinline__ void zero_moments_body(const CellArgs &a, const int lb, const int gb) {
     uint32_t l = lb * blockDim.x + threadIdx.x;
     if (l >= *a.n_labels) return;
     for (int q = 1; q < a.cw; ++q) a.state[(size_t)l * a.cw + q] = 0;
@@ -156,7 +167,7 @@ __device__ __forceinline__ void zero_moments_body(CellArgs a, const int lb, cons
 
 // Move each live label's anchor to its rounded centroid, then clear its sums
 // for a moments-only recount. Gated on flags[gate] when gate >= 0.
-__device__ __forceinline__ void recentre_body(CellArgs a, int gate, const int lb, const int gb) {
+__device__ __forceinline__ void recentre_body(const CellArgs &a, int gate, const int lb, const int gb) {
     if (gate >= 0 && a.flags[gate] == 0) return;
     uint32_t l = lb * blockDim.x + threadIdx.x;
     if (l >= *a.n_labels) return;
@@ -178,7 +189,7 @@ __device__ __forceinline__ void recentre_body(CellArgs a, int gate, const int lb
 
 // Whether some live label's centroid sits more than an eighth of a side from
 // its anchor; sets flags[1].
-__device__ __forceinline__ void check_recentre_body(CellArgs a, const int lb, const int gb) {
+__device__ __forceinline__ void check_recentre_body(const CellArgs &a, const int lb, const int gb) {
     uint32_t l = lb * blockDim.x + threadIdx.x;
     if (l == 0 || l >= *a.n_labels) return;
     const int64_t *s = a.state + (size_t)l * a.cw;
@@ -201,7 +212,7 @@ __device__ __forceinline__ void check_recentre_body(CellArgs a, const int lb, co
 // Decide each live label's fate: divide at its division volume (cut through
 // the centroid across the major axis), else die with the type's rate by a
 // counter draw of (step, label). Sets flags[0] when anything happens.
-__device__ __forceinline__ void plan_body(CellArgs a, const int lb, const int gb) {
+__device__ __forceinline__ void plan_body(const CellArgs &a, const int lb, const int gb) {
     uint32_t l = lb * blockDim.x + threadIdx.x;
     uint32_t n = *a.n_labels;
     if (l >= n) return;
@@ -237,7 +248,7 @@ __device__ __forceinline__ bool beyond_cut(const CellArgs &a, uint32_t l, int x,
 }
 
 // Sites each planned division would move.
-__device__ __forceinline__ void count_moved_body(CellArgs a, size_t n_sites, const int lb, const int gb) {
+__device__ __forceinline__ void count_moved_body(const CellArgs &a, size_t n_sites, const int lb, const int gb) {
     if (a.flags[0] == 0) return;
     size_t i = (size_t)lb * blockDim.x + threadIdx.x;
     if (i >= n_sites) return;
@@ -251,7 +262,7 @@ __device__ __forceinline__ void count_moved_body(CellArgs a, size_t n_sites, con
 // One workgroup: cancel divisions that would move nothing or everything,
 // hand out daughter ids in ascending parent order, and copy types and
 // anchors to the daughters.
-__device__ __forceinline__ void assign_daughters_body(CellArgs a, const int lb, const int gb) {
+__device__ __forceinline__ void assign_daughters_body(const CellArgs &a, const int lb, const int gb) {
     if (a.flags[0] == 0) return;
     __shared__ uint32_t part[1024];
     __shared__ uint32_t base;
@@ -299,7 +310,7 @@ __device__ __forceinline__ void assign_daughters_body(CellArgs a, const int lb, 
 
 // Relabel: a dying label's sites become medium, a dividing label's sites
 // beyond the cut go to its daughter.
-__device__ __forceinline__ void apply_population_body(CellArgs a, size_t n_sites, const int lb, const int gb) {
+__device__ __forceinline__ void apply_population_body(const CellArgs &a, size_t n_sites, const int lb, const int gb) {
     if (a.flags[0] == 0) return;
     size_t i = (size_t)lb * blockDim.x + threadIdx.x;
     if (i >= n_sites) return;
@@ -320,7 +331,7 @@ __device__ __forceinline__ void apply_population_body(CellArgs a, size_t n_sites
 }
 
 // Zero the per-label state ahead of a full recount, gated like recount.
-__device__ __forceinline__ void zero_state_body(CellArgs a, int gate, const int lb, const int gb) {
+__device__ __forceinline__ void zero_state_body(const CellArgs &a, int gate, const int lb, const int gb) {
     if (gate >= 0 && a.flags[gate] == 0) return;
     size_t n = (size_t)(*a.n_labels) * a.cw;
     for (size_t i = (size_t)lb * blockDim.x + threadIdx.x; i < n;
@@ -331,7 +342,7 @@ __device__ __forceinline__ void zero_state_body(CellArgs a, int gate, const int 
 // One workgroup: when more than max(64, (n-1)/4) labels are dead, renumber
 // the live ones densely in ascending order (remap into scratch_u32) and set
 // flags[3].
-__device__ __forceinline__ void compact_plan_body(CellArgs a, const int lb, const int gb) {
+__device__ __forceinline__ void compact_plan_body(const CellArgs &a, const int lb, const int gb) {
     if (a.flags[0] == 0) return;
     __shared__ uint32_t part[1024];
     __shared__ uint32_t dead_total;
@@ -373,7 +384,7 @@ __device__ __forceinline__ void compact_plan_body(CellArgs a, const int lb, cons
 // Move types and anchors of live labels to their new ids (ascending, so a
 // label only ever moves down and no source is overwritten before it is read
 // when done in one thread; the parallel form reads from a staged copy).
-__device__ __forceinline__ void compact_move_body(CellArgs a, const uint32_t *type_copy, const int32_t *anchor_copy, const int lb, const int gb) {
+__device__ __forceinline__ void compact_move_body(const CellArgs &a, const uint32_t *type_copy, const int32_t *anchor_copy, const int lb, const int gb) {
     if (a.flags[0] == 0 || a.flags[3] == 0) return;
     uint32_t l = lb * blockDim.x + threadIdx.x;
     uint32_t n = *a.n_labels;
@@ -386,7 +397,7 @@ __device__ __forceinline__ void compact_move_body(CellArgs a, const uint32_t *ty
     a.anchor[3 * to + 2] = anchor_copy[3 * l + 2];
 }
 
-__device__ __forceinline__ void compact_relabel_body(CellArgs a, size_t n_sites, const int lb, const int gb) {
+__device__ __forceinline__ void compact_relabel_body(const CellArgs &a, size_t n_sites, const int lb, const int gb) {
     if (a.flags[0] == 0 || a.flags[3] == 0) return;
     size_t i = (size_t)lb * blockDim.x + threadIdx.x;
     if (i >= n_sites) return;
@@ -396,7 +407,7 @@ __device__ __forceinline__ void compact_relabel_body(CellArgs a, size_t n_sites,
     a.labels[i] = (word & ~kIdMask) | a.scratch_u32[l];
 }
 
-__device__ __forceinline__ void compact_finish_body(CellArgs a, const int lb, const int gb) {
+__device__ __forceinline__ void compact_finish_body(const CellArgs &a, const int lb, const int gb) {
     if (a.flags[0] == 0 || a.flags[3] == 0) return;
     if (lb == 0 && threadIdx.x == 0) *a.n_labels = a.flags[3];
 }

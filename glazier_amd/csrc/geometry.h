@@ -12,6 +12,38 @@
 
 namespace ghip {
 
+// One Jacobi rotation in the (P, R) plane: zero a[P][R] of the symmetric
+// 3x3 `a`, applying the same rotation to the eigenvector columns `v`. The
+// plane is a template argument so every index is a constant and the
+// matrices stay in registers.
+template <int P, int R>
+GHD void jacobi_rotate(double (&a)[3][3], double (&v)[3][3]) {
+    const double apq = a[P][R];
+    if (apq == 0.0) return;
+    const double tau = (a[R][R] - a[P][P]) / (2.0 * apq);
+    const double t = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+    const double c = 1.0 / sqrt(1.0 + t * t);
+    const double s = t * c;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const double akp = a[k][P], akr = a[k][R];
+        a[k][P] = c * akp - s * akr;
+        a[k][R] = s * akp + c * akr;
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const double apk = a[P][k], ark = a[R][k];
+        a[P][k] = c * apk - s * ark;
+        a[R][k] = s * apk + c * ark;
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const double vkp = v[k][P], vkr = v[k][R];
+        v[k][P] = c * vkp - s * vkr;
+        v[k][R] = s * vkp + c * vkr;
+    }
+}
+
 // Moment word order after volume/surface: sx, sy, sz, sxx, syy, szz, sxy,
 // sxz, syz.
 GHD double cov_largest(const int64_t *m, int64_t n, bool three_d, double *axis) {
@@ -48,43 +80,29 @@ GHD double cov_largest(const int64_t *m, int64_t n, bool three_d, double *axis) 
     double cyz = (double)(n * m[8] - m[1] * m[2]);
     double a[3][3] = {{cxx, cxy, cxz}, {cxy, cyy, cyz}, {cxz, cyz, czz}};
     double v[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
-    const int pp[3] = {0, 0, 1};
-    const int rr[3] = {1, 2, 2};
+#pragma unroll
     for (int sweep = 0; sweep < 5; ++sweep) {
-        for (int q = 0; q < 3; ++q) {
-            int p = pp[q], r = rr[q];
-            double apq = a[p][r];
-            if (apq == 0.0) continue;
-            double tau = (a[r][r] - a[p][p]) / (2.0 * apq);
-            double t = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
-            double c = 1.0 / sqrt(1.0 + t * t);
-            double s = t * c;
-            for (int k = 0; k < 3; ++k) {
-                double akp = a[k][p], akr = a[k][r];
-                a[k][p] = c * akp - s * akr;
-                a[k][r] = s * akp + c * akr;
-            }
-            for (int k = 0; k < 3; ++k) {
-                double apk = a[p][k], ark = a[r][k];
-                a[p][k] = c * apk - s * ark;
-                a[r][k] = s * apk + c * ark;
-            }
-            for (int k = 0; k < 3; ++k) {
-                double vkp = v[k][p], vkr = v[k][r];
-                v[k][p] = c * vkp - s * vkr;
-                v[k][r] = s * vkp + c * vkr;
-            }
-        }
+        jacobi_rotate<0, 1>(a, v);
+        jacobi_rotate<0, 2>(a, v);
+        jacobi_rotate<1, 2>(a, v);
     }
-    int best = 0;
-    if (a[1][1] > a[best][best]) best = 1;
-    if (a[2][2] > a[best][best]) best = 2;
+    // the largest diagonal entry, the first of equals
+    double best = a[0][0];
+    int col = 0;
+    if (a[1][1] > best) {
+        best = a[1][1];
+        col = 1;
+    }
+    if (a[2][2] > best) {
+        best = a[2][2];
+        col = 2;
+    }
     if (axis) {
-        axis[0] = v[0][best];
-        axis[1] = v[1][best];
-        axis[2] = v[2][best];
+        axis[0] = col == 0 ? v[0][0] : col == 1 ? v[0][1] : v[0][2];
+        axis[1] = col == 0 ? v[1][0] : col == 1 ? v[1][1] : v[1][2];
+        axis[2] = col == 0 ? v[2][0] : col == 1 ? v[2][1] : v[2][2];
     }
-    return a[best][best] > 0.0 ? a[best][best] : 0.0;
+    return best > 0.0 ? best : 0.0;
 }
 
 // Major axis 4 sqrt(lambda_max(cov)) of n sites with sums m.

@@ -152,6 +152,14 @@ def case(name):
         m.types[1].target_volume = 512.0
         l, t = lone_cell(m, 4, 12)
         return m, l, t, None, None, 4
+    if name == "length3d":
+        m = ref_model(16, 16, 16, seed=61, order=3, target_length=14.0, lambda_length=2.0)
+        m.types[1].target_volume = 512.0
+        return (m, *tiled(m), None, None, 6)
+    if name == "division3d":
+        m = ref_model(16, 16, 16, seed=62, order=3, division_volume=72.0, death_rate=0.01)
+        m.types[1].target_volume = 100.0
+        return (m, *tiled(m, side=4), None, None, 10)
     raise KeyError(name)
 
 
@@ -159,4 +167,5 @@ PARITY_CASES = ["plain128", "surface256", "surface1024", "fast_medium_flat", "fa
                 "fast_lone_typed", "ref64", "medium_gaps", "rect64x32",
                 "von_neumann", "float_energies", "volume3d_o2", "volume3d_o3", "volume3d_o1",
                 "connected", "connected3d", "chemotaxis_ramp", "secreting", "length",
-                "motility", "drift", "death", "division", "infection128", "motility3d"]
+                "motility", "drift", "death", "division", "infection128", "motility3d",
+                "length3d", "division3d"]

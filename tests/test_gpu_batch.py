@@ -18,7 +18,7 @@ pytestmark = pytest.mark.gpu
 BATCH_CASES = ["plain128", "fast_medium_typed", "medium_gaps", "rect64x32", "von_neumann",
                "float_energies", "volume3d_o2", "volume3d_o3", "volume3d_o1", "connected",
                "connected3d", "chemotaxis_ramp", "secreting", "length", "motility", "drift",
-               "death", "division", "infection128", "motility3d"]
+               "death", "division", "infection128", "motility3d", "length3d", "division3d"]
 
 
 def assert_same(name, m, gpu, emu):
