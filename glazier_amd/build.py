@@ -18,8 +18,8 @@ CSRC = HERE / "csrc"
 OUT = HERE / "libglazier_hip.so"
 CLI = HERE / "glazier"
 CLI_SOURCES = ["cli.cpp", "blueprint.cpp"]
-SOURCES = ["sweep.hip", "sweep_plain.hip", "fields.hip", "cells.hip", "batch.hip", "engine.cpp",
-           "batch_host.cpp"]
+SOURCES = ["sweep.hip", "sweep_plain.hip", "fields.hip", "cells.hip", "batch.hip", "adjoint.hip",
+           "engine.cpp", "batch_host.cpp", "fit_host.cpp"]
 OBJ_DIR = HERE / "_build"
 FLAGS = [
     "--offload-arch=gfx950",

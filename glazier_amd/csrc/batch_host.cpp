@@ -532,6 +532,9 @@ ghip_status ghip_batch_set_tiles(ghip_batch *b, int candidate) {
 
 ghip_status ghip_batch_step(ghip_batch *b, uint64_t steps) {
     if (!b) return fail(GHIP_BAD_ARG, "null handle");
+    for (ghip_sim *s : b->sims)
+        if (s->traj) return fail(GHIP_UNSUPPORTED, "a lattice in the batch is recording its trajectory; "
+                                                   "step it alone");
     bool stale = false;
     for (size_t r = 0; r < b->sims.size(); ++r) {
         HIP_TRY(hipStreamSynchronize(b->sims[r]->stream));

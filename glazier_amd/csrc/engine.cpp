@@ -17,6 +17,7 @@
 #include <vector>
 
 #include "../../include/glazier_hip.h"
+#include "fit.h"
 #include "ghip_device.h"
 #include "sim.h"
 
@@ -539,12 +540,20 @@ ghip_status ghip_destroy(ghip_sim *sim) {
 
 ghip_status ghip_step(ghip_sim *s, uint64_t steps) {
     if (!s) return fail(GHIP_BAD_ARG, "null handle");
+    if (s->traj && s->traj_len + steps > s->traj_cap)
+        return fail(GHIP_BAD_ARG, "the recording has room for " + std::to_string(s->traj_cap - s->traj_len) +
+                                      " more steps");
     for (uint64_t q = 0; q < steps; ++q) {
         int order[8];
         colour_order(s->m.seed, s->m.replica, s->mcs, s->colours, order);
         for (int c = 0; c < s->colours; ++c) {
             ghip_status st = run_sweep_pass(s, order[c]);
             if (st != GHIP_OK) return st;
+        }
+        if (s->traj) {
+            // the owners this step's field exchange reads
+            HIP_TRY(launch_record_types(s->labels[s->lab_cur], s->traj + s->traj_len * s->N, s->N, s->stream));
+            s->traj_len += 1;
         }
         if (s->m.n_species) {
             ghip_status st = run_fields(s);

@@ -96,11 +96,15 @@ struct ghip_sim {
 
     ghip_batch *batch = nullptr;  // the batch this lattice is stepped in, if any
 
+    // owner-type trajectory: one byte per site per recorded step
+    uint8_t *traj = nullptr;
+    uint64_t traj_cap = 0, traj_len = 0;
+
     ~ghip_sim() {
         if (stream) (void)hipStreamSynchronize(stream);
         void *bufs[] = {labels[0], labels[1], act[0], act[1], field_base[0], field_base[1], A[0],
                         A[1], Dl[0], Dl[1], Dl[2], anchor, ltype, scratch, daughter, plan, geo,
-                        flags, type_copy, anchor_copy, n_labels_dev, tables, accepted};
+                        flags, type_copy, anchor_copy, n_labels_dev, tables, accepted, traj};
         for (void *p : bufs)
             if (p) (void)hipFree(p);
         for (auto *list : {&pending, &spare})

@@ -46,6 +46,14 @@ def library() -> ctypes.CDLL:
     if not LIB_PATH.exists():
         raise ImportError(
             f"{LIB_PATH} is not built; run __graft_entry__.build() (hipcc for gfx950)")
+    # One HIP runtime per process: PyTorch-ROCm ships its own libamdhip64
+    # (same soname as ROCm's). Loading torch first lets the engine's
+    # dependency resolve to the runtime already in the process; the other
+    # order leaves two runtimes, and torch then finds no device.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = ctypes.CDLL(str(LIB_PATH))
     P = ctypes.POINTER
     sim = ctypes.c_void_p

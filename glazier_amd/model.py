@@ -236,3 +236,87 @@ def default_cell_types(n_labels: int, types: Sequence[int] | None = None) -> np.
         for label, kind in enumerate(types):
             out[label + 1] = kind
     return out
+
+
+class CFieldParams(ctypes.Structure):
+    """`ghip_field_params` of include/glazier_hip.h."""
+
+    _fields_ = [
+        ("diffusion", ctypes.c_double * MAX_SPECIES),
+        ("decay", ctypes.c_double * MAX_SPECIES),
+        ("secretion", (ctypes.c_double * MAX_SPECIES) * MAX_TYPES),
+        ("uptake", (ctypes.c_double * MAX_SPECIES) * MAX_TYPES),
+    ]
+
+
+@dataclass
+class FieldParams:
+    """The rates a field replay is a function of: per species diffusion and
+    decay (per step, split over the substeps as the engine splits them), per
+    (cell type, species) secretion and uptake (Fields::exchange,
+    glazier-core/src/field.rs:133-149). Row 0 of the per-type tables is the
+    medium, which exchanges nothing."""
+
+    diffusion: np.ndarray
+    decay: np.ndarray
+    secretion: np.ndarray
+    uptake: np.ndarray
+
+    @property
+    def n_species(self) -> int:
+        return int(self.diffusion.size)
+
+    @property
+    def n_types(self) -> int:
+        return int(self.secretion.shape[0])
+
+    @staticmethod
+    def from_model(model: "Model") -> "FieldParams":
+        s = len(model.species)
+        return FieldParams(
+            diffusion=np.array([sp.diffusion for sp in model.species], dtype=np.float64),
+            decay=np.array([sp.decay for sp in model.species], dtype=np.float64),
+            secretion=np.array(model.secretion, dtype=np.float64).reshape(model.n_types, s),
+            uptake=np.array(model.uptake, dtype=np.float64).reshape(model.n_types, s))
+
+    def to_c(self) -> CFieldParams:
+        c = CFieldParams()
+        for k in range(self.n_species):
+            c.diffusion[k] = float(self.diffusion[k])
+            c.decay[k] = float(self.decay[k])
+            for t in range(self.n_types):
+                c.secretion[t][k] = float(self.secretion[t, k])
+                c.uptake[t][k] = float(self.uptake[t, k])
+        return c
+
+    @staticmethod
+    def from_c(c: CFieldParams, n_types: int, n_species: int) -> "FieldParams":
+        return FieldParams(
+            diffusion=np.array([c.diffusion[k] for k in range(n_species)], dtype=np.float64),
+            decay=np.array([c.decay[k] for k in range(n_species)], dtype=np.float64),
+            secretion=np.array([[c.secretion[t][k] for k in range(n_species)]
+                                for t in range(n_types)], dtype=np.float64),
+            uptake=np.array([[c.uptake[t][k] for k in range(n_species)]
+                             for t in range(n_types)], dtype=np.float64))
+
+    def vector(self) -> np.ndarray:
+        """diffusion, decay, then the cell types' secretion and uptake rows
+        (the medium row left out), as one f64 vector."""
+        return np.concatenate([self.diffusion, self.decay, self.secretion[1:].ravel(),
+                               self.uptake[1:].ravel()])
+
+    @staticmethod
+    def from_vector(v, n_types: int, n_species: int) -> "FieldParams":
+        v = np.asarray(v, dtype=np.float64)
+        s, c = n_species, (n_types - 1) * n_species
+        zero = np.zeros((1, s))
+        return FieldParams(
+            diffusion=v[:s].copy(), decay=v[s:2 * s].copy(),
+            secretion=np.vstack([zero, v[2 * s:2 * s + c].reshape(n_types - 1, s)]),
+            uptake=np.vstack([zero, v[2 * s + c:2 * s + 2 * c].reshape(n_types - 1, s)]))
+
+    def names(self) -> List[str]:
+        s, t = self.n_species, self.n_types
+        return ([f"diffusion[{k}]" for k in range(s)] + [f"decay[{k}]" for k in range(s)]
+                + [f"secretion[{a}][{k}]" for a in range(1, t) for k in range(s)]
+                + [f"uptake[{a}][{k}]" for a in range(1, t) for k in range(s)])

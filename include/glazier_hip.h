@@ -192,6 +192,56 @@ ghip_status ghip_batch_set_tiles(ghip_batch *batch, int candidate);
 /* A JSON description of the groups, their chosen tiles and timings. */
 ghip_status ghip_batch_describe(const ghip_batch *batch, char *out, size_t n);
 
+/* ---- Field replay and its adjoint. No counterpart in the reference, which
+ * has no derivative of any kind; the forward step replayed here is the
+ * engine's field step (Fields::diffuse and Fields::exchange,
+ * glazier-core/src/field.rs:99-149; field_diffuse and field_exchange,
+ * glazier-cuda/src/lib.rs:314-363).
+ *
+ * While recording is on, every ghip_step stores each step's cell-type map
+ * (one byte per site, 0 for the medium) after its colour passes: the owners
+ * the field exchange of that step used. A fit takes such a trajectory and
+ * replays the field steps over it, holding the labels fixed, as a function
+ * of the field parameters and the initial field; its backward pass gives
+ * the gradient of a loss on the final fields with respect to both. */
+typedef struct ghip_field_params {
+    double diffusion[GHIP_MAX_SPECIES];
+    double decay[GHIP_MAX_SPECIES];
+    double secretion[GHIP_MAX_TYPES][GHIP_MAX_SPECIES];
+    double uptake[GHIP_MAX_TYPES][GHIP_MAX_SPECIES];
+} ghip_field_params;
+
+/* Starts recording with room for `max_steps` steps (0 stops and frees it). */
+ghip_status ghip_record(ghip_sim *sim, uint64_t max_steps);
+ghip_status ghip_recorded_steps(const ghip_sim *sim, uint64_t *out);
+/* The type map of recorded step `step` (n = n_sites). */
+ghip_status ghip_recorded_types(const ghip_sim *sim, uint64_t step, uint8_t *out, size_t n);
+
+typedef struct ghip_fit ghip_fit;
+
+/* A replay of the recorded steps of `sim` (copied: the lattice may go on or
+ * be destroyed), keeping the fields of every `checkpoint_every`-th step in
+ * the forward pass and recomputing the rest in the backward pass. */
+ghip_status ghip_fit_create(const ghip_sim *sim, uint32_t checkpoint_every, ghip_fit **out);
+ghip_status ghip_fit_destroy(ghip_fit *fit);
+
+/* The fit's own parameters as the model states them (a starting point). */
+ghip_status ghip_fit_model_params(const ghip_fit *fit, ghip_field_params *out);
+
+/* Forward replay. `u0` and `u_final` are device pointers to n_species *
+ * n_sites f32 values, species-major; `stream` is the hipStream_t to order
+ * the work on (NULL for the fit's own, synchronised before returning). The
+ * f32 arithmetic is the engine's, so the replay of a lattice's own
+ * parameters reproduces its fields bit for bit. */
+ghip_status ghip_fit_forward(ghip_fit *fit, const ghip_field_params *params, const float *u0,
+                             float *u_final, void *stream);
+
+/* Backward pass of the last forward: `grad_final` (device, like u_final) is
+ * dL/du_final; writes dL/dparams (accumulated in f64) to `grad` on the host
+ * and dL/du0 to the device buffer `grad_u0`. Synchronises `stream`. */
+ghip_status ghip_fit_backward(ghip_fit *fit, const float *grad_final, ghip_field_params *grad,
+                              float *grad_u0, void *stream);
+
 /* Message of the last failure on this thread; empty when none. */
 const char *ghip_last_error(void);
 

@@ -8,7 +8,11 @@ emu.c by oracle/Makefile):
                     (glazier-core/src/cpu.rs): the distributional reference
                     and the timed CPU baseline;
   EmuSimulation     the CPU emulation of the device checkerboard schedule:
-                    the bit-exact reference for the HIP kernels.
+                    the bit-exact reference for the HIP kernels;
+  replay_forward / replay_gradient
+                    the f64 replay of the field step over a recorded type
+                    trajectory and its adjoint (adjoint.c): the reference
+                    for the device gradient.
 """
 
 from __future__ import annotations
@@ -20,7 +24,7 @@ from typing import Optional
 
 import numpy as np
 
-from glazier_amd.model import CModel, Model
+from glazier_amd.model import CFieldParams, CModel, FieldParams, Model
 
 HERE = Path(__file__).resolve().parent
 LIB_PATH = HERE / "liboracle.so"
@@ -104,6 +108,11 @@ def library() -> ctypes.CDLL:
         "emu_delta_energy": ([v, sz, sz], f64),
         "emu_fields_step": ([v], None),
         "emu_one_piece": ([u32, ctypes.c_int], ctypes.c_int),
+        "adj_forward": ([ctypes.c_int, ctypes.c_int, ctypes.c_int, u32, P(ctypes.c_int),
+                         P(CFieldParams), P(ctypes.c_uint8), u64, P(f64), P(f64)], None),
+        "adj_gradient": ([ctypes.c_int, ctypes.c_int, ctypes.c_int, u32, P(ctypes.c_int),
+                          P(CFieldParams), P(ctypes.c_uint8), u64, P(f64), P(f64),
+                          P(CFieldParams), P(f64)], None),
     }
     for name, (args, res) in sigs.items():
         fn = getattr(lib, name)
@@ -343,3 +352,40 @@ def hash32(x: int) -> int:
 
 def one_piece(mask: int, ring_size: int) -> bool:
     return bool(library().emu_one_piece(mask, ring_size))
+
+
+def _substeps(model: Model):
+    return np.array([library().serial_substeps(k, ctypes.byref(model.to_c()))
+                     for k in range(len(model.species))], dtype=np.int32)
+
+
+def replay_forward(model: Model, params: FieldParams, types: np.ndarray,
+                   u0: np.ndarray) -> np.ndarray:
+    """f64 fields after replaying `types` (steps x n_sites, 0 = medium) from
+    `u0` (n_species x n_sites) with `params` (adjoint.c, adj_forward)."""
+    types = np.ascontiguousarray(types, dtype=np.uint8).reshape(-1, model.sites)
+    u0 = np.ascontiguousarray(u0, dtype=np.float64).reshape(len(model.species), -1)
+    sub = _substeps(model)
+    out = np.empty_like(u0)
+    c = params.to_c()
+    library().adj_forward(model.width, model.height, model.depth, len(model.species),
+                          _p(sub, ctypes.c_int), ctypes.byref(c), _p(types, ctypes.c_uint8),
+                          types.shape[0], _p(u0, ctypes.c_double), _p(out, ctypes.c_double))
+    return out
+
+
+def replay_gradient(model: Model, params: FieldParams, types: np.ndarray, u0: np.ndarray,
+                    grad_out: np.ndarray):
+    """(dL/dparams, dL/du0) of a loss with dL/du_final = grad_out, in f64
+    (adjoint.c, adj_gradient)."""
+    types = np.ascontiguousarray(types, dtype=np.uint8).reshape(-1, model.sites)
+    u0 = np.ascontiguousarray(u0, dtype=np.float64).reshape(len(model.species), -1)
+    g = np.ascontiguousarray(grad_out, dtype=np.float64).reshape(u0.shape)
+    sub = _substeps(model)
+    gu0 = np.empty_like(u0)
+    c, gc = params.to_c(), CFieldParams()
+    library().adj_gradient(model.width, model.height, model.depth, len(model.species),
+                           _p(sub, ctypes.c_int), ctypes.byref(c), _p(types, ctypes.c_uint8),
+                           types.shape[0], _p(u0, ctypes.c_double), _p(g, ctypes.c_double),
+                           ctypes.byref(gc), _p(gu0, ctypes.c_double))
+    return FieldParams.from_c(gc, params.n_types, params.n_species), gu0

@@ -4,14 +4,16 @@
  * TEST INFRASTRUCTURE ONLY. Nothing in the engine links, loads or calls this
  * code; the tests, the smoke check and the CPU baseline of the benchmark do.
  *
- * Two oracles live here:
+ * Three oracles live here:
  *   serial_*  a restatement of glazier-core's serial f64 engine
  *             (glazier-core/src/cpu.rs, rng.rs, field.rs, moments.rs,
  *             connectivity.rs, motility.rs, lattice.rs): the distributional
  *             reference and the timed CPU baseline;
  *   emu_*     a CPU emulation of the device's checkerboard schedule: the same
  *             colour order, counter hash, per-colour snapshots and f32/int
- *             arithmetic as the HIP kernels, the bit-exact reference.
+ *             arithmetic as the HIP kernels, the bit-exact reference;
+ *   adj_*     an f64 replay of the field step over a recorded trajectory and
+ *             its adjoint, the reference for the device gradient.
  */
 #ifndef GLAZIER_ORACLE_H
 #define GLAZIER_ORACLE_H
@@ -99,6 +101,18 @@ void emu_activity_f32(const emu_sim *e, float *out);
 double emu_delta_energy(const emu_sim *e, size_t target, size_t source);
 void emu_fields_step(emu_sim *e);
 int emu_one_piece(uint32_t present, int ring_size);
+
+/* ---------------------------------------------------------------- adjoint */
+/* f64 replay of the field step over a recorded type trajectory (`steps`
+ * maps of n_sites bytes) and its adjoint (adjoint.c). Fields are
+ * species-major n_species * n_sites values. */
+void adj_forward(int w, int h, int d, uint32_t n_species, const int *substeps,
+                 const ghip_field_params *p, const uint8_t *types, uint64_t steps,
+                 const double *u0, double *out);
+void adj_gradient(int w, int h, int d, uint32_t n_species, const int *substeps,
+                  const ghip_field_params *p, const uint8_t *types, uint64_t steps,
+                  const double *u0, const double *grad_out, ghip_field_params *grad,
+                  double *grad_u0);
 
 #ifdef __cplusplus
 }
