@@ -15,7 +15,9 @@ The same line carries "ensemble": the batched-versus-looped comparison on
 the uneven ensemble (64 infection sheets of 64^2..512^2 plus 16 volumes of
 64^3, parameters drawn by seed 3), each lattice run for --ensemble-mcs steps
 once as one batch and once lattice by lattice; replicas/s is lattices
-finished per second of wall time, setup excluded.
+finished per second of wall time, setup excluded. And "adjoint": the
+gradient of the 256^2 field fit (forward and backward times, agreement with
+the f64 CPU adjoint, and 50 Adam steps on the perturbed rates).
 """
 
 from __future__ import annotations
@@ -131,6 +133,91 @@ def ensemble_bench(mcs: int):
     }
 
 
+def adjoint_bench(cpu: bool, adam_steps: int = 50):
+    """The differentiated field fit: a 256^2 infection sheet (1024 cells,
+    seed 7) run for 200 steps with its owner trajectory recorded; targets
+    from the replay with perturbed rates (D x1.3, decay x0.7, secretion x1.2,
+    uptake x0.8); the gradient timed, checked against the f64 CPU adjoint,
+    and used by Adam to bring the rates back."""
+    import numpy as np
+    import torch
+
+    from glazier_amd import HipSimulation
+    from glazier_amd.adjoint import FieldFit, fit_rates, record, recorded_types
+    from glazier_amd.model import FieldParams
+    from glazier_amd.workloads import infection_sheet
+
+    steps, every = 200, 20
+    model, labels, types = infection_sheet(256, 7)
+    sim = HipSimulation(model, labels, types)
+    record(sim, steps)
+    sim.step(steps)
+    fit = FieldFit(sim, checkpoint_every=every)
+    base = fit.model_params()
+    truth = FieldParams(base.diffusion * 1.3, base.decay * 0.7, base.secretion * 1.2, base.uptake * 0.8)
+    u0 = torch.zeros((2, model.sites), dtype=torch.float32, device="cuda")
+    target = fit.forward(truth, u0)
+    # gradient timing: forward + backward of the L2 loss
+    for _ in range(2):
+        out = fit.forward(base, u0)
+        fit.backward(2.0 * (out - target))
+    torch.cuda.synchronize()
+    reps = 5
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        out = fit.forward(base, u0)
+    torch.cuda.synchronize()
+    fwd = (time.perf_counter() - t0) / reps
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        grad, _ = fit.backward(2.0 * (out - target))
+    torch.cuda.synchronize()
+    bwd = (time.perf_counter() - t0) / reps
+    substeps = 4
+    # algorithmic bytes of one replayed step: per species and substep one f32
+    # read and one f32 write per site, plus the exchange's read, write and
+    # one-byte owner read
+    n = model.sites
+    fwd_bytes = steps * 2 * n * (substeps * 8 + 8) + steps * n
+    line = {
+        "workload": "256^2 infection sheet, 1024 cells, seed 7, 200 steps recorded, "
+                    "checkpoints every 20; rates perturbed D x1.3, decay x0.7, secretion x1.2, "
+                    "uptake x0.8",
+        "forward_ms": fwd * 1e3, "gradient_ms": (fwd + bwd) * 1e3, "backward_ms": bwd * 1e3,
+        "forward_gbs_algorithmic": fwd_bytes / fwd / 1e9,
+        "parameters": base.names(),
+    }
+    t0 = time.perf_counter()
+    hist = fit_rates(fit, target, base, u0, steps=adam_steps, lr=0.2)
+    torch.cuda.synchronize()
+    free = base.vector() > 0
+    rel = np.abs(hist[-1]["theta"][free] / truth.vector()[free] - 1.0)
+    line["adam"] = {"steps": adam_steps, "lr": 0.2, "seconds": time.perf_counter() - t0,
+                    "loss_start": hist[0]["loss"], "loss_end": hist[-1]["loss"],
+                    "rate_rel_error_max": float(rel.max()), "rate_rel_error_mean": float(rel.mean()),
+                    "start_rel_error_mean": float(np.abs(base.vector()[free] / truth.vector()[free]
+                                                         - 1.0).mean())}
+    if cpu:
+        import oracle
+
+        traj = np.stack([recorded_types(sim, s) for s in range(steps)])
+        u0h = np.zeros((2, n))
+        tgt64 = oracle.replay_forward(model, truth, traj, u0h)
+        t0 = time.perf_counter()
+        out64 = oracle.replay_forward(model, base, traj, u0h)
+        want, _ = oracle.replay_gradient(model, base, traj, u0h, 2.0 * (out64 - tgt64))
+        cpu_s = time.perf_counter() - t0
+        gv, wv = grad.vector(), want.vector()
+        scale = np.abs(wv).max()
+        line["cpu_f64_gradient_s"] = cpu_s
+        line["cpu_threads"] = 1
+        line["gradient_vs_f64_adjoint_max_rel"] = float(
+            max(abs(gv[i] - wv[i]) / max(abs(wv[i]), 1e-3 * scale) for i in range(gv.size)))
+    fit.close()
+    sim.close()
+    return line
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -141,6 +228,7 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--ensemble-mcs", type=int, default=100)
     ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy a MIB-mebibyte buffer 5 times with torch, so a counter "
                          "run can scale FETCH_SIZE/WRITE_SIZE by a copy of known size")
@@ -231,6 +319,8 @@ def main() -> int:
         s.close()
     if not args.no_ensemble:
         line["ensemble"] = ensemble_bench(args.ensemble_mcs)
+    if not args.no_adjoint:
+        line["adjoint"] = adjoint_bench(cpu=not args.no_cpu)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(side, args.cpu_seconds)
     print(json.dumps(line))

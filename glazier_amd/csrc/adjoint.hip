@@ -20,8 +20,11 @@
 //     dL/ds[t] += pass * lambda'' * (1 - up),   dL/dup[t] -= pass * lambda'' * (u + s)
 //
 // Adjoint fields are f32 like the fields; every parameter derivative is
-// formed and summed in f64, per workgroup first and then with one f64 atomic
-// per workgroup and parameter.
+// formed and summed in f64. Each workgroup adds its sum into its own slot of
+// a partials buffer (a launch's grid is the same on every step, so slot b
+// collects block b's share of every step, in step order, with no atomics and
+// no contention); one reduction over the slots ends the backward pass. The
+// gradient is therefore the same on every run.
 #include "fit.h"
 
 namespace ghip {
@@ -113,12 +116,13 @@ __global__ __launch_bounds__(256) void fit_exchange_kernel(FitGrid g, FitCoef c,
     out[(size_t)k * g.n + i] = val;
 }
 
-// Transposed substep s of every species; gacc[2k] += dL/dd_k, gacc[2k+1] += dL/dk_k.
+// Transposed substep s of every species. Block (b, k) adds dL/dd_k and
+// dL/dk_k into part[(k * blocks + b) * kPartWords + {0, 1}].
 template <bool D3>
 __global__ __launch_bounds__(256) void fit_substep_adjoint_kernel(FitGrid g, FitCoef c,
                                                                   const float *lam_out,
                                                                   const float *u_before, float *lam_in,
-                                                                  double *gacc, int s) {
+                                                                  double *part, int s) {
     __shared__ double red[4];
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const int k = blockIdx.y;
@@ -144,23 +148,25 @@ __global__ __launch_bounds__(256) void fit_substep_adjoint_kernel(FitGrid g, Fit
     gd = block_sum(gd, red);
     gk = block_sum(gk, red);
     if (threadIdx.x == 0) {
-        atomicAdd(gacc + 2 * k, gd);
-        atomicAdd(gacc + 2 * k + 1, gk);
+        double *slot = part + ((size_t)k * gridDim.x + blockIdx.x) * kPartWords;
+        slot[0] += gd;
+        slot[1] += gk;
     }
 }
 
-// Transposed exchange, in place on lam; per-type sums go to
-// gacc[kExchangeBase + 2 * (t * kMaxSpecies + k) + {0: secretion, 1: uptake}].
+// Transposed exchange, in place on lam. Block (b, k) adds its per-type sums
+// into part[(k * blocks + b) * kPartWords + 2 + 2 * t + {0: secretion, 1: uptake}].
 __global__ __launch_bounds__(256) void fit_exchange_adjoint_kernel(FitGrid g, FitCoef c, const uint8_t *types,
                                                                    const float *u_pre, float *lam,
-                                                                   double *gacc) {
-    __shared__ double acc[kMaxTypes][2];
+                                                                   double *part) {
+    __shared__ double acc[4][kMaxTypes][2];  // per wave
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const int k = blockIdx.y;
-    for (int q = threadIdx.x; q < 2 * kMaxTypes; q += blockDim.x) acc[q >> 1][q & 1] = 0.0;
-    __syncthreads();
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    uint32_t t = 0;
+    double gs = 0.0, gu = 0.0;
     if (i < g.n) {
-        const uint32_t t = types[i];
+        t = types[i];
         if (t != 0) {
             const size_t at = (size_t)k * g.n + i;
             const float up = c.up[t][k];
@@ -170,17 +176,45 @@ __global__ __launch_bounds__(256) void fit_exchange_adjoint_kernel(FitGrid g, Fi
             if (w < 0.0f) {
                 lam[at] = 0.0f;
             } else {
-                atomicAdd(&acc[t][0], (double)lo * (1.0 - (double)up));
-                atomicAdd(&acc[t][1], -(double)lo * (double)v);
+                gs = (double)lo * (1.0 - (double)up);
+                gu = -(double)lo * (double)v;
                 lam[at] = lo - up * lo;
             }
         }
     }
-    __syncthreads();
-    for (int q = threadIdx.x; q < 2 * g.n_types; q += blockDim.x) {
-        const double val = acc[q >> 1][q & 1];
-        if (val != 0.0) atomicAdd(gacc + kExchangeBase + 2 * ((q >> 1) * kMaxSpecies + k) + (q & 1), val);
+    // one wave reduction per cell type (the medium exchanges nothing)
+    for (int q = 1; q < g.n_types; ++q) {
+        double a = t == (uint32_t)q ? gs : 0.0, b = t == (uint32_t)q ? gu : 0.0;
+        for (int off = 32; off > 0; off >>= 1) {
+            a += __shfl_xor(a, off, 64);
+            b += __shfl_xor(b, off, 64);
+        }
+        if (lane == 0) {
+            acc[wave][q][0] = a;
+            acc[wave][q][1] = b;
+        }
     }
+    __syncthreads();
+    double *slot = part + ((size_t)k * gridDim.x + blockIdx.x) * kPartWords + 2;
+    const int words = 2 * g.n_types, waves = (int)(blockDim.x >> 6);
+    for (int q = threadIdx.x; q < words; q += blockDim.x) {
+        if ((q >> 1) == 0) continue;
+        double v = 0.0;
+        for (int wv = 0; wv < waves; ++wv) v += acc[wv][q >> 1][q & 1];
+        slot[q] += v;
+    }
+}
+
+// Sum of the partials of every block: grad[k * kPartWords + w] for species k
+// and word w, one workgroup per (word, species), in block order.
+__global__ __launch_bounds__(256) void fit_reduce_kernel(const double *part, uint32_t blocks, double *grad) {
+    __shared__ double red[4];
+    const int w = blockIdx.x, k = blockIdx.y;
+    double v = 0.0;
+    for (uint32_t b = threadIdx.x; b < blocks; b += blockDim.x)
+        v += part[((size_t)k * blocks + b) * kPartWords + w];
+    v = block_sum(v, red);
+    if (threadIdx.x == 0) grad[k * kPartWords + w] = v;
 }
 
 // ---------------------------------------------------------------- launchers
@@ -211,21 +245,29 @@ hipError_t launch_fit_exchange(const FitGrid &g, const FitCoef &c, uint32_t spec
 
 hipError_t launch_fit_substep_adjoint(const FitGrid &g, const FitCoef &c, uint32_t species,
                                       const float *lam_out, const float *u_before, float *lam_in,
-                                      double *gacc, int s, hipStream_t st) {
+                                      double *part, int s, hipStream_t st) {
     if (g.d3)
         hipLaunchKernelGGL(fit_substep_adjoint_kernel<true>, site_grid(g, species), dim3(256), 0, st, g, c,
-                           lam_out, u_before, lam_in, gacc, s);
+                           lam_out, u_before, lam_in, part, s);
     else
         hipLaunchKernelGGL(fit_substep_adjoint_kernel<false>, site_grid(g, species), dim3(256), 0, st, g, c,
-                           lam_out, u_before, lam_in, gacc, s);
+                           lam_out, u_before, lam_in, part, s);
     return hipGetLastError();
 }
 
 hipError_t launch_fit_exchange_adjoint(const FitGrid &g, const FitCoef &c, uint32_t species,
-                                       const uint8_t *types, const float *u_pre, float *lam, double *gacc,
+                                       const uint8_t *types, const float *u_pre, float *lam, double *part,
                                        hipStream_t st) {
     hipLaunchKernelGGL(fit_exchange_adjoint_kernel, site_grid(g, species), dim3(256), 0, st, g, c, types, u_pre,
-                       lam, gacc);
+                       lam, part);
+    return hipGetLastError();
+}
+
+uint32_t fit_blocks(const FitGrid &g) { return (g.n + 255) / 256; }
+
+hipError_t launch_fit_reduce(const FitGrid &g, uint32_t species, const double *part, double *grad,
+                             hipStream_t st) {
+    hipLaunchKernelGGL(fit_reduce_kernel, dim3(kPartWords, species), dim3(256), 0, st, part, fit_blocks(g), grad);
     return hipGetLastError();
 }
 

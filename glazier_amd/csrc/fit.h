@@ -22,10 +22,11 @@ struct FitCoef {
     int sub[kMaxSpecies];
 };
 
-// Gradient accumulator layout (f64): [2k] dL/dd_k, [2k + 1] dL/dk_k, then
-// from kExchangeBase, per (type t, species k), dL/dsec and dL/dup.
-constexpr int kExchangeBase = 2 * kMaxSpecies;
-constexpr int kGradWords = kExchangeBase + 2 * kMaxTypes * kMaxSpecies;
+// Per-species gradient words (f64): [0] dL/dd, [1] dL/dk, then per cell type
+// t, [2 + 2t] dL/dsec[t] and [3 + 2t] dL/dup[t]. The partials buffer holds
+// these words for every (species, workgroup) of a site launch; the reduced
+// gradient holds them per species.
+constexpr int kPartWords = 2 + 2 * kMaxTypes;
 
 hipError_t launch_record_types(const uint32_t *labels, uint8_t *out, size_t n, hipStream_t s);
 hipError_t launch_fit_substep(const FitGrid &g, const FitCoef &c, uint32_t species, const float *in,
@@ -34,9 +35,13 @@ hipError_t launch_fit_exchange(const FitGrid &g, const FitCoef &c, uint32_t spec
                                const float *in, float *out, hipStream_t st);
 hipError_t launch_fit_substep_adjoint(const FitGrid &g, const FitCoef &c, uint32_t species,
                                       const float *lam_out, const float *u_before, float *lam_in,
-                                      double *gacc, int s, hipStream_t st);
+                                      double *part, int s, hipStream_t st);
 hipError_t launch_fit_exchange_adjoint(const FitGrid &g, const FitCoef &c, uint32_t species,
-                                       const uint8_t *types, const float *u_pre, float *lam, double *gacc,
+                                       const uint8_t *types, const float *u_pre, float *lam, double *part,
                                        hipStream_t st);
+// Workgroups of a site launch, and the reduction of the partials.
+uint32_t fit_blocks(const FitGrid &g);
+hipError_t launch_fit_reduce(const FitGrid &g, uint32_t species, const double *part, double *grad,
+                             hipStream_t st);
 
 }  // namespace ghip

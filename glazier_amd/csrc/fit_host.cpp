@@ -34,7 +34,8 @@ struct ghip_fit {
     float *sub = nullptr;        // max_sub * S * n (substep states 1..max_sub)
     float *work[2] = {nullptr, nullptr};
     float *lam[2] = {nullptr, nullptr};
-    double *gacc = nullptr;
+    double *part = nullptr;      // kPartWords per (species, workgroup)
+    double *grad = nullptr;      // kPartWords per species
     hipStream_t own = nullptr;
     FitCoef coef{};
     bool have_forward = false;
@@ -43,7 +44,7 @@ struct ghip_fit {
 
     ~ghip_fit() {
         if (own) (void)hipStreamSynchronize(own);
-        void *bufs[] = {types, ckpt, seg, sub, work[0], work[1], lam[0], lam[1], gacc};
+        void *bufs[] = {types, ckpt, seg, sub, work[0], work[1], lam[0], lam[1], part, grad};
         for (void *p : bufs)
             if (p) (void)hipFree(p);
         if (own) (void)hipStreamDestroy(own);
@@ -155,7 +156,8 @@ ghip_status ghip_fit_create(const ghip_sim *s, uint32_t checkpoint_every, ghip_f
         FIT_TRY(hipMalloc(&f->work[i], fw * 4));
         FIT_TRY(hipMalloc(&f->lam[i], fw * 4));
     }
-    FIT_TRY(hipMalloc(&f->gacc, kGradWords * sizeof(double)));
+    FIT_TRY(hipMalloc(&f->part, (size_t)f->S * fit_blocks(f->g) * kPartWords * sizeof(double)));
+    FIT_TRY(hipMalloc(&f->grad, (size_t)f->S * kPartWords * sizeof(double)));
     FIT_TRY(hipStreamSynchronize(s->stream));
     FIT_TRY(hipMemcpy(f->types, s->traj, f->steps * s->N, hipMemcpyDeviceToDevice));
 #undef FIT_TRY
@@ -211,7 +213,7 @@ ghip_status ghip_fit_backward(ghip_fit *f, const float *grad_final, ghip_field_p
     if (!f->have_forward) return fail(GHIP_BAD_ARG, "backward needs a forward pass first");
     hipStream_t st = stream ? (hipStream_t)stream : f->own;
     const size_t fw = f->field_words();
-    HIP_TRY(hipMemsetAsync(f->gacc, 0, kGradWords * sizeof(double), st));
+    HIP_TRY(hipMemsetAsync(f->part, 0, (size_t)f->S * fit_blocks(f->g) * kPartWords * sizeof(double), st));
     HIP_TRY(hipMemcpyAsync(f->lam[0], grad_final, fw * 4, hipMemcpyDeviceToDevice, st));
     int li = 0;
     const uint64_t n_seg = (f->steps + f->K - 1) / f->K;
@@ -234,27 +236,29 @@ ghip_status ghip_fit_backward(ghip_fit *f, const float *grad_final, ghip_field_p
             }
             const float *pre = f->sub + (size_t)(f->max_sub - 1) * fw;
             HIP_TRY(launch_fit_exchange_adjoint(f->g, f->coef, f->S, f->types + m * f->g.n, pre, f->lam[li],
-                                                f->gacc, st));
+                                                f->part, st));
             for (int s = f->max_sub; s >= 1; --s) {
                 const float *before = s == 1 ? start : f->sub + (size_t)(s - 2) * fw;
                 HIP_TRY(launch_fit_substep_adjoint(f->g, f->coef, f->S, f->lam[li], before, f->lam[li ^ 1],
-                                                   f->gacc, s, st));
+                                                   f->part, s, st));
                 li ^= 1;
             }
         }
     }
     HIP_TRY(hipMemcpyAsync(grad_u0, f->lam[li], fw * 4, hipMemcpyDeviceToDevice, st));
-    double acc[kGradWords];
-    HIP_TRY(hipMemcpyAsync(acc, f->gacc, sizeof acc, hipMemcpyDeviceToHost, st));
+    HIP_TRY(launch_fit_reduce(f->g, f->S, f->part, f->grad, st));
+    std::vector<double> acc((size_t)f->S * kPartWords);
+    HIP_TRY(hipMemcpyAsync(acc.data(), f->grad, acc.size() * sizeof(double), hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
     std::memset(grad, 0, sizeof *grad);
     for (uint32_t k = 0; k < f->S; ++k) {
+        const double *w = acc.data() + (size_t)k * kPartWords;
         const double sub = (double)f->m.species[k].substeps;
-        grad->diffusion[k] = acc[2 * k] / sub;
-        grad->decay[k] = acc[2 * k + 1] / sub;
+        grad->diffusion[k] = w[0] / sub;
+        grad->decay[k] = w[1] / sub;
         for (uint32_t t = 0; t < f->m.n_types; ++t) {
-            grad->secretion[t][k] = acc[kExchangeBase + 2 * (t * kMaxSpecies + k)];
-            grad->uptake[t][k] = acc[kExchangeBase + 2 * (t * kMaxSpecies + k) + 1];
+            grad->secretion[t][k] = w[2 + 2 * t];
+            grad->uptake[t][k] = w[3 + 2 * t];
         }
     }
     return GHIP_OK;
