@@ -197,25 +197,34 @@ def adjoint_bench(cpu: bool, adam_steps: int = 50):
                     "rate_rel_error_max": float(rel.max()), "rate_rel_error_mean": float(rel.mean()),
                     "start_rel_error_mean": float(np.abs(base.vector()[free] / truth.vector()[free]
                                                          - 1.0).mean())}
-    if cpu:
-        import oracle
-
-        traj = np.stack([recorded_types(sim, s) for s in range(steps)])
-        u0h = np.zeros((2, n))
-        tgt64 = oracle.replay_forward(model, truth, traj, u0h)
-        t0 = time.perf_counter()
-        out64 = oracle.replay_forward(model, base, traj, u0h)
-        want, _ = oracle.replay_gradient(model, base, traj, u0h, 2.0 * (out64 - tgt64))
-        cpu_s = time.perf_counter() - t0
-        gv, wv = grad.vector(), want.vector()
-        scale = np.abs(wv).max()
-        line["cpu_f64_gradient_s"] = cpu_s
-        line["cpu_threads"] = 1
-        line["gradient_vs_f64_adjoint_max_rel"] = float(
-            max(abs(gv[i] - wv[i]) / max(abs(wv[i]), 1e-3 * scale) for i in range(gv.size)))
+    # kept for the CPU leg: the trajectory and the device gradient
+    line["_cpu_leg"] = {"model": model, "truth": truth, "base": base, "grad": grad,
+                        "traj": np.stack([recorded_types(sim, k) for k in range(steps)]) if cpu else None}
     fit.close()
     sim.close()
     return line
+
+
+def cpu_baseline_adjoint(leg: dict):
+    """The f64 CPU adjoint (the oracle, one thread) of the same 256^2 fit:
+    its time for one gradient, and how far the device gradient sits from it."""
+    import numpy as np
+
+    import oracle
+
+    model, truth, base, grad, traj = leg["model"], leg["truth"], leg["base"], leg["grad"], leg["traj"]
+    u0 = np.zeros((2, model.sites))
+    target = oracle.replay_forward(model, truth, traj, u0)
+    t0 = time.perf_counter()
+    out = oracle.replay_forward(model, base, traj, u0)
+    want, _ = oracle.replay_gradient(model, base, traj, u0, 2.0 * (out - target))
+    seconds = time.perf_counter() - t0
+    gv, wv = grad.vector(), want.vector()
+    scale = np.abs(wv).max()
+    return {"value": seconds, "unit": "s per gradient", "cores": 1, "kind": "port",
+            "sample": "f64 replay and adjoint oracle, the whole 256^2 200-step fit, one thread",
+            "device_vs_f64_max_rel": float(max(abs(gv[i] - wv[i]) / max(abs(wv[i]), 1e-3 * scale)
+                                               for i in range(gv.size)))}
 
 
 def main() -> int:
@@ -319,10 +328,14 @@ def main() -> int:
         s.close()
     if not args.no_ensemble:
         line["ensemble"] = ensemble_bench(args.ensemble_mcs)
+    leg = None
     if not args.no_adjoint:
         line["adjoint"] = adjoint_bench(cpu=not args.no_cpu)
+        leg = line["adjoint"].pop("_cpu_leg")
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(side, args.cpu_seconds)
+        if leg is not None:
+            line["cpu_baseline"]["adjoint"] = cpu_baseline_adjoint(leg)
     print(json.dumps(line))
     return 0
 
