@@ -44,10 +44,8 @@ __device__ __forceinline__ uint32_t unequal8(uint32_t v, uint32_t n0, uint32_t n
 
 __global__ __launch_bounds__(256) void sweep_plain2d_kernel(SweepArgs a, PlainParams p) {
     __shared__ __attribute__((aligned(16))) uint32_t tile[kRows * kPitch];
-    __shared__ uint32_t qtl[kPTargets];
-    __shared__ uint32_t qr[kPTargets];
+    __shared__ uint4 queue[kPTargets];  // (tile offset, r24, old label, new label)
     __shared__ uint32_t thr_s[kPThr];
-    __shared__ int delta_s[8];
     __shared__ uint32_t qcount;
     __shared__ uint32_t acc_block;
 
@@ -64,12 +62,6 @@ __global__ __launch_bounds__(256) void sweep_plain2d_kernel(SweepArgs a, PlainPa
     if (tid == 0) {
         qcount = 0;
         acc_block = 0;
-    }
-    if (tid < 8) {
-        // LDS offset of pick k: ORDER3's planar order, (dx, dy) row by row
-        const int q = tid + (tid >> 2);
-        const int dy = q / 3 - 1, dx = q % 3 - 1;
-        delta_s[tid] = dy * kPitch + dx;
     }
 
     // ---- issue every global read of the workgroup before the first wait.
@@ -151,9 +143,12 @@ __global__ __launch_bounds__(256) void sweep_plain2d_kernel(SweepArgs a, PlainPa
         const uint32_t site = srow + 2u * (uint32_t)lane;
         const int tl = tlrow + 2 * lane;
         const uint32_t h = site_draw(a.key, a.key2, site);
-        const uint32_t pick = h & 7u;
+        // pick k in ORDER3's planar order, (dx, dy) row by row: dx + 1 and
+        // dy + 1 are the k-th 2-bit fields of 0x9224 and 0xA940
+        const uint32_t sh = (h & 7u) * 2u;
+        const int dxy = (int)(((0xA940u >> sh) & 3u) * (uint32_t)kPitch + ((0x9224u >> sh) & 3u));
         const uint32_t old = tile[tl];
-        const uint32_t nw = tile[tl + delta_s[pick]];
+        const uint32_t nw = tile[tl - kPitch - 1 + dxy];
         const unsigned long long m = __ballot(old != nw);
         if (m == 0) continue;
         uint32_t base = 0;
@@ -162,8 +157,7 @@ __global__ __launch_bounds__(256) void sweep_plain2d_kernel(SweepArgs a, PlainPa
         if (old != nw) {
             const uint32_t pos = base + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
                                                                   __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-            qtl[pos] = (uint32_t)tl | (pick << 16);
-            qr[pos] = h >> 8;
+            queue[pos] = make_uint4((uint32_t)tl, h >> 8, old, nw);
         }
     }
     __syncthreads();
@@ -173,27 +167,27 @@ __global__ __launch_bounds__(256) void sweep_plain2d_kernel(SweepArgs a, PlainPa
     const uint32_t cw = (uint32_t)a.cell_words;
     uint32_t my_acc = 0;
     for (uint32_t qi = tid; qi < nq; qi += 256) {
-        const uint32_t e = qtl[qi];
-        const uint32_t r24 = qr[qi];
-        const int tl = (int)(e & 0xFFFFu);
-        const uint32_t pick = e >> 16;
-        const uint32_t old = tile[tl];
-        const uint32_t nw = tile[tl + delta_s[pick]];
+        const uint4 e = queue[qi];
+        const int tl = (int)e.x;
+        const uint32_t r24 = e.y, old = e.z, nw = e.w;
         const uint32_t oid = label_id(old), nid = label_id(nw);
-        // four independent gathers; label 0's record is real memory whose
-        // value is never used
-        const int64_t pv_old = a.a_cur[oid * cw] + a.e_prev[oid * cw];
-        const int64_t pv_new = a.a_cur[nid * cw] + a.e_prev[nid * cw];
+        // the four gathers go out together, before anything waits on them;
+        // label 0's record is real memory whose value is never used
+        const int64_t ao = a.a_cur[oid * cw], eo = a.e_prev[oid * cw];
+        const int64_t an = a.a_cur[nid * cw], en = a.e_prev[nid * cw];
         const uint32_t n0 = tile[tl - kPitch - 1], n1 = tile[tl - kPitch], n2 = tile[tl - kPitch + 1];
         const uint32_t n3 = tile[tl - 1], n4 = tile[tl + 1];
         const uint32_t n5 = tile[tl + kPitch - 1], n6 = tile[tl + kPitch], n7 = tile[tl + kPitch + 1];
-        // absolute states keep surface in [0, 2^31): volume is the high word
-        const int vol_old = (int)(uint32_t)((uint64_t)pv_old >> 32);
-        const int vol_new = (int)(uint32_t)((uint64_t)pv_new >> 32);
-        if (oid != 0 && vol_old <= 1) continue;
         const int unlike_old = (int)unequal8(old, n0, n1, n2, n3, n4, n5, n6, n7);
         const int unlike_new = (int)unequal8(nw, n0, n1, n2, n3, n4, n5, n6, n7);
         const int like_old = 8 - unlike_old, like_new = 8 - unlike_new;
+        const int64_t pv_old = ao + eo, pv_new = an + en;
+        // absolute states keep surface in [0, 2^31): volume is the high word
+        const int vol_old = (int)(uint32_t)((uint64_t)pv_old >> 32);
+        const int vol_new = (int)(uint32_t)((uint64_t)pv_new >> 32);
+        // a copy that would empty a cell is refused (no branch before the
+        // gathers are used, so none of them waits behind another)
+        const bool emptying = oid != 0 && vol_old <= 1;
         int med = 0;
         if (has_medium) med = 8 - (int)unequal8(0u, n0, n1, n2, n3, n4, n5, n6, n7);
         // contact gained minus lost, from counts (exact: integer energies)
@@ -229,7 +223,7 @@ __global__ __launch_bounds__(256) void sweep_plain2d_kernel(SweepArgs a, PlainPa
             const float dsf = (float)ds_new;
             dh = dh + p.lam_s * (dsf * (2.0f * (s - p.tgt_s) + dsf));
         }
-        const bool acc = dh <= 0.0f || (dh < (float)p.n_thr && r24 < thr_s[(int)dh]);
+        const bool acc = !emptying && (dh <= 0.0f || (dh < (float)p.n_thr && r24 < thr_s[(int)dh]));
         if (!acc) continue;
         const int ly = tl / kPitch, lx = tl - ly * kPitch;
         a.labels[(uint32_t)(y0 + ly - 1) * (uint32_t)W + (uint32_t)(x0 + lx - 4)] = nw;
