@@ -197,15 +197,17 @@ void choose_tiles(ghip_sim *s) {
     }
     s->threads = ((s->threads + 63) / 64) * 64;
     s->blocks = s->tiles_x * s->tiles_y * s->tiles_z;
+    int px = 0, py = 0;
+    plain2d_tile(s->W, s->H, &px, &py);
     s->plain_fast = !s->d3 && s->n_off == 8 && s->m.n_types == 2 &&
-                    (s->sweep_flags & kIntegral) && !s->full && s->W % 128 == 0 && s->H % 16 == 0 &&
+                    (s->sweep_flags & kIntegral) && !s->full && px > 0 &&
                     plain2d_supported(s->host_tables.n_thr);
     if (s->plain_fast) {
-        s->tiles_x = s->W / 128;
-        s->tiles_y = s->H / 16;
+        s->tiles_x = s->W / px;
+        s->tiles_y = s->H / py;
         s->tiles_z = 1;
         s->blocks = s->tiles_x * s->tiles_y;
-        s->threads = 256;
+        s->threads = px == 256 ? 512 : 256;
     }
     // fields: 64 x 32 interior in a plane, 16 x 8 x 8 in a volume
     if (!s->d3) {

@@ -134,6 +134,8 @@ size_t sweep_lds_bytes(const SweepArgs &a, bool d3);
 hipError_t launch_sweep_plain2d(const SweepArgs &a, const PlainParams &p, int blocks,
                                 hipStream_t stream);
 bool plain2d_supported(uint32_t n_thr);
+// Tile of the plain pass for a w x h lattice (0 x 0: the pass does not apply).
+void plain2d_tile(int w, int h, int *px, int *py);
 hipError_t launch_field(const FieldArgs &a, bool d3, int blocks, hipStream_t stream);
 size_t field_lds_bytes(const FieldArgs &a, bool d3);
 hipError_t launch_activity_decay(float *act, size_t n, hipStream_t stream);

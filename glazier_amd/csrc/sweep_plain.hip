@@ -3,33 +3,33 @@
 // integer energies (contact, volume, surface). This is the throughput path.
 //
 // Same draws, same decisions and the same bookkeeping as sweep_kernel (and so
-// as the CPU emulation); only the work is organised for gfx950, where this
-// pass is bound by VALU issue rather than by bytes:
+// as the CPU emulation); only the work is organised for gfx950:
 //
-//  * a 256-lane workgroup owns 128 x 16 sites. Every global read of the
-//    workgroup (18 tile rows as 16-byte loads, the halo columns, the
+//  * a workgroup owns a PX x PY tile (256 x 32 sites with 512 lanes where the
+//    lattice divides, else 128 x 16 with 256 lanes). Every global read of the
+//    workgroup (the PY + 2 tile rows as 16-byte loads, the halo columns, the
 //    acceptance table and its share of the snapshot fold) is issued before
-//    the first wait; rows land in an LDS tile padded to a 136-word pitch
+//    the first wait; rows land in an LDS tile padded to a PX + 8 word pitch
 //    (interior at column 4, so each 16-byte LDS write is aligned);
 //  * phase A draws every target. A wavefront's 64 targets share one tile row,
 //    so row bases stay in scalar registers; the target's label and its picked
 //    neighbour come from LDS, and the targets whose label would change are
-//    appended to an LDS queue (one LDS atomic per wavefront). In a confluent
-//    sheet three targets in four have old == new and cost nothing more;
-//  * phase B prices the queued targets densely. With integer energies the
-//    contact sum is exact in any order, so it comes from neighbour counts
-//    taken as min(a ^ b, 1) sums (no compare-to-mask hazards), and the medium
-//    count is skipped for tiles that hold no medium.
+//    appended to an LDS queue of 16-bit target indices (one LDS atomic per
+//    wavefront). In a confluent sheet most targets have old == new and cost
+//    nothing more;
+//  * phase B prices the queued targets densely: it redraws the target's hash
+//    (cheaper than holding it in LDS, which would cost residency), issues the
+//    four per-label gathers together, and takes the contact sum from
+//    neighbour counts (exact with integer energies); the medium count is
+//    skipped for tiles that hold no medium.
+//
+// Larger tiles carry more loads per workgroup lifetime and amortise the
+// fixed per-workgroup work; they matter most once copies are accepted.
 #include "ghip_device.h"
 
 namespace ghip {
 
-constexpr int kPX = 128;
-constexpr int kPY = 16;
-constexpr int kPitch = 136;
-constexpr int kRows = kPY + 2;
-constexpr int kPTargets = kPX * kPY / 4;
-constexpr int kPThr = 256;
+constexpr int kPThr = 256;  // acceptance-table entries held in LDS
 
 namespace {
 // Count of neighbours differing from v: sum of min(n ^ v, 1).
@@ -42,21 +42,30 @@ __device__ __forceinline__ uint32_t unequal8(uint32_t v, uint32_t n0, uint32_t n
 }
 }  // namespace
 
-__global__ __launch_bounds__(256) void sweep_plain2d_kernel(SweepArgs a, PlainParams p) {
+template <int PX, int PY, int NT>
+__global__ __launch_bounds__(NT) void sweep_plain2d_kernel(SweepArgs a, PlainParams p) {
+    constexpr int kPitch = PX + 8;
+    constexpr int kRows = PY + 2;
+    constexpr int kTargets = PX * PY / 4;
+    constexpr int kC4 = PX / 4;                          // 16-byte loads per row
+    constexpr int kLoads = (kRows * kC4 + NT - 1) / NT;  // per lane
+    constexpr int kTRow = PX / 2;                        // targets per target row
+    static_assert(kTRow % 64 == 0, "a wavefront's targets share one row");
+    static_assert(kTargets % NT == 0 && kTargets <= 65536, "targets per lane, 16-bit queue");
+
     __shared__ __attribute__((aligned(16))) uint32_t tile[kRows * kPitch];
-    __shared__ uint4 queue[kPTargets];  // (tile offset, r24, old label, new label)
+    __shared__ uint16_t queue[kTargets];
     __shared__ uint32_t thr_s[kPThr];
     __shared__ uint32_t qcount;
     __shared__ uint32_t acc_block;
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
-    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     int b = blockIdx.x;
     if (a.xcd_chunk > 0) b = (b & 7) * a.xcd_chunk + (b >> 3);
     const int tile_x = b % a.tiles_x;
     const int tile_y = b / a.tiles_x;
-    const int x0 = tile_x * kPX, y0 = tile_y * kPY;
+    const int x0 = tile_x * PX, y0 = tile_y * PY;
     const int W = a.w, H = a.h;
 
     if (tid == 0) {
@@ -64,25 +73,21 @@ __global__ __launch_bounds__(256) void sweep_plain2d_kernel(SweepArgs a, PlainPa
         acc_block = 0;
     }
 
-    // ---- issue every global read of the workgroup before the first wait.
-    // A wavefront loads two tile rows per step: lanes 0-31 the first, lanes
-    // 32-63 the second; the row offsets are scalars chosen per half-wave.
-    const int half = lane >> 5;
-    const uint32_t col4 = (uint32_t)(lane & 31) * 4u;
-    uint4 v[3];
-    int vdst[3];
+    // ---- issue every global read of the workgroup before the first wait
+    uint4 v[kLoads];
+    int vdst[kLoads];
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        const int r0 = 8 * k + 2 * wave;  // uniform
+    for (int k = 0; k < kLoads; ++k) {
+        const int idx = k * NT + tid;
+        const int r = idx / kC4, c4 = idx % kC4;
         vdst[k] = -1;
-        if (r0 < kRows) {
-            int g0 = y0 - 1 + r0, g1 = g0 + 1;
-            g0 += g0 < 0 ? H : 0;
-            g0 -= g0 >= H ? H : 0;
-            g1 -= g1 >= H ? H : 0;
-            const uint32_t row = (uint32_t)(half ? g1 : g0) * (uint32_t)W + (uint32_t)x0;
-            v[k] = *reinterpret_cast<const uint4 *>(a.labels + row + col4);
-            vdst[k] = (r0 + half) * kPitch + 4 + (int)col4;
+        if (r < kRows) {
+            int g = y0 - 1 + r;
+            g += g < 0 ? H : 0;
+            g -= g >= H ? H : 0;
+            v[k] = *reinterpret_cast<const uint4 *>(a.labels + (uint32_t)g * (uint32_t)W + (uint32_t)x0 +
+                                                    4u * (uint32_t)c4);
+            vdst[k] = r * kPitch + 4 + 4 * c4;
         }
     }
     uint32_t hv = 0;
@@ -92,11 +97,11 @@ __global__ __launch_bounds__(256) void sweep_plain2d_kernel(SweepArgs a, PlainPa
         int gy = y0 - 1 + r;
         gy += gy < 0 ? H : 0;
         gy -= gy >= H ? H : 0;
-        int gx = right ? x0 + kPX : x0 - 1;
+        int gx = right ? x0 + PX : x0 - 1;
         gx += gx < 0 ? W : 0;
         gx -= gx >= W ? W : 0;
         hv = a.labels[(uint32_t)gy * (uint32_t)W + (uint32_t)gx];
-        hpos = r * kPitch + (right ? 4 + kPX : 3);
+        hpos = r * kPitch + (right ? 4 + PX : 3);
     }
     uint32_t tv = 0;
     if (tid < (int)p.n_thr) tv = p.thr[tid];
@@ -113,7 +118,7 @@ __global__ __launch_bounds__(256) void sweep_plain2d_kernel(SweepArgs a, PlainPa
     // ---- land them
     uint32_t lo = 0xFFFFFFFFu;
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
+    for (int k = 0; k < kLoads; ++k) {
         if (vdst[k] >= 0) {
             *reinterpret_cast<uint4 *>(tile + vdst[k]) = v[k];
             if (p.needs_medium) lo = min(lo, min(min(v[k].x, v[k].y), min(v[k].z, v[k].w)));
@@ -128,27 +133,36 @@ __global__ __launch_bounds__(256) void sweep_plain2d_kernel(SweepArgs a, PlainPa
         a.a_next[fi] = fa + fe;
         a.g_zero[fi] = 0;
     }
-    for (uint32_t i = fi + 256; i < fhi; i += 256) {
+    for (uint32_t i = fi + NT; i < fhi; i += NT) {
         a.a_next[i] = a.a_cur[i] + a.e_prev[i];
         a.g_zero[i] = 0;
     }
     const bool has_medium = __syncthreads_or(p.needs_medium && lo == 0u) != 0;
 
-    // ---- phase A: draw every target, queue the ones whose label would change
-#pragma unroll
-    for (int k = 0; k < kPTargets / 256; ++k) {
-        const int j = wave + 4 * k;  // target row, uniform in the wavefront
-        const uint32_t srow = (uint32_t)(y0 + 2 * j + a.cy) * (uint32_t)W + (uint32_t)(x0 + a.cx);
-        const int tlrow = (1 + 2 * j + a.cy) * kPitch + 4 + a.cx;
-        const uint32_t site = srow + 2u * (uint32_t)lane;
-        const int tl = tlrow + 2 * lane;
-        const uint32_t h = site_draw(a.key, a.key2, site);
-        // pick k in ORDER3's planar order, (dx, dy) row by row: dx + 1 and
-        // dy + 1 are the k-th 2-bit fields of 0x9224 and 0xA940
+    // target q of the tile: row j (uniform in a wavefront), column i
+    auto target = [&](int q, uint32_t *site, int *tl) {
+        const int j = q / kTRow, i = q % kTRow;
+        *site = (uint32_t)(y0 + 2 * j + a.cy) * (uint32_t)W + (uint32_t)(x0 + a.cx + 2 * i);
+        *tl = (1 + 2 * j + a.cy) * kPitch + 4 + a.cx + 2 * i;
+    };
+    // pick k in ORDER3's planar order, (dx, dy) row by row: dx + 1 and dy + 1
+    // are the k-th 2-bit fields of 0x9224 and 0xA940
+    auto picked = [&](uint32_t h, int tl) {
         const uint32_t sh = (h & 7u) * 2u;
         const int dxy = (int)(((0xA940u >> sh) & 3u) * (uint32_t)kPitch + ((0x9224u >> sh) & 3u));
+        return tile[tl - kPitch - 1 + dxy];
+    };
+
+    // ---- phase A: draw every target, queue the ones whose label would change
+#pragma unroll
+    for (int k = 0; k < kTargets / NT; ++k) {
+        const int q = k * NT + tid;
+        uint32_t site;
+        int tl;
+        target(q, &site, &tl);
+        const uint32_t h = site_draw(a.key, a.key2, site);
         const uint32_t old = tile[tl];
-        const uint32_t nw = tile[tl - kPitch - 1 + dxy];
+        const uint32_t nw = picked(h, tl);
         const unsigned long long m = __ballot(old != nw);
         if (m == 0) continue;
         uint32_t base = 0;
@@ -157,7 +171,7 @@ __global__ __launch_bounds__(256) void sweep_plain2d_kernel(SweepArgs a, PlainPa
         if (old != nw) {
             const uint32_t pos = base + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
                                                                   __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-            queue[pos] = make_uint4((uint32_t)tl, h >> 8, old, nw);
+            queue[pos] = (uint16_t)q;
         }
     }
     __syncthreads();
@@ -166,10 +180,14 @@ __global__ __launch_bounds__(256) void sweep_plain2d_kernel(SweepArgs a, PlainPa
     const uint32_t nq = qcount;
     const uint32_t cw = (uint32_t)a.cell_words;
     uint32_t my_acc = 0;
-    for (uint32_t qi = tid; qi < nq; qi += 256) {
-        const uint4 e = queue[qi];
-        const int tl = (int)e.x;
-        const uint32_t r24 = e.y, old = e.z, nw = e.w;
+    for (uint32_t qi = tid; qi < nq; qi += NT) {
+        uint32_t site;
+        int tl;
+        target(queue[qi], &site, &tl);
+        const uint32_t h = site_draw(a.key, a.key2, site);
+        const uint32_t r24 = h >> 8;
+        const uint32_t old = tile[tl];
+        const uint32_t nw = picked(h, tl);
         const uint32_t oid = label_id(old), nid = label_id(nw);
         // the four gathers go out together, before anything waits on them;
         // label 0's record is real memory whose value is never used
@@ -225,8 +243,7 @@ __global__ __launch_bounds__(256) void sweep_plain2d_kernel(SweepArgs a, PlainPa
         }
         const bool acc = !emptying && (dh <= 0.0f || (dh < (float)p.n_thr && r24 < thr_s[(int)dh]));
         if (!acc) continue;
-        const int ly = tl / kPitch, lx = tl - ly * kPitch;
-        a.labels[(uint32_t)(y0 + ly - 1) * (uint32_t)W + (uint32_t)(x0 + lx - 4)] = nw;
+        a.labels[site] = nw;
         if (oid != 0)
             atomicAdd(reinterpret_cast<unsigned long long *>(a.f_acc + oid * cw),
                       (unsigned long long)pack_vs(-1, ds_old));
@@ -243,9 +260,28 @@ __global__ __launch_bounds__(256) void sweep_plain2d_kernel(SweepArgs a, PlainPa
 
 bool plain2d_supported(uint32_t n_thr) { return n_thr <= (uint32_t)kPThr; }
 
+void plain2d_tile(int w, int h, int *px, int *py) {
+    if (w % 256 == 0 && h % 32 == 0) {
+        *px = 256;
+        *py = 32;
+    } else if (w % 128 == 0 && h % 16 == 0) {
+        *px = 128;
+        *py = 16;
+    } else {
+        *px = *py = 0;
+    }
+}
+
 hipError_t launch_sweep_plain2d(const SweepArgs &a, const PlainParams &p, int blocks,
                                 hipStream_t stream) {
-    hipLaunchKernelGGL(sweep_plain2d_kernel, dim3(blocks), dim3(256), 0, stream, a, p);
+    int px = 0, py = 0;
+    plain2d_tile(a.w, a.h, &px, &py);
+    if (px == 256)
+        hipLaunchKernelGGL((sweep_plain2d_kernel<256, 32, 512>), dim3(blocks), dim3(512), 0, stream, a, p);
+    else if (px == 128)
+        hipLaunchKernelGGL((sweep_plain2d_kernel<128, 16, 256>), dim3(blocks), dim3(256), 0, stream, a, p);
+    else
+        return hipErrorInvalidValue;
     return hipGetLastError();
 }
 
