@@ -227,6 +227,55 @@ def cpu_baseline_adjoint(leg: dict):
                                                for i in range(gv.size)))}
 
 
+def counter_per_dispatch(args, counter: str, kernel: str):
+    """Run this script under `rocprofv3 --pmc <counter>` in a child process
+    (the program directly after `--`) and return the counter's mean value per
+    dispatch of `kernel`, summed over the counter's instances; None when the
+    profiler is missing or the run fails."""
+    import shutil
+    import sqlite3
+    import subprocess
+    import tempfile
+
+    prof = shutil.which("rocprofv3")
+    if not prof:
+        return None
+    with tempfile.TemporaryDirectory() as out:
+        cmd = ["timeout", "-s", "KILL", "120", prof, "--pmc", counter, "-d", out, "-o", "run", "--",
+               sys.executable, str(ROOT / "bench.py"), *args]
+        done = subprocess.run(cmd, capture_output=True, text=True)
+        dbs = sorted(Path(out).rglob("*.db"))
+        if not dbs:
+            return None
+        con = sqlite3.connect(str(dbs[0]))
+        rows = con.execute("select dispatch_id, sum(counter_value) from pmc_events where name like ? and "
+                           "counter_name = ? group by dispatch_id", (f"%{kernel}%", counter)).fetchall()
+        con.close()
+        del done
+    if not rows:
+        return None
+    return sum(r[1] for r in rows) / len(rows)
+
+
+def measured_traffic(side: int):
+    """HBM bytes per launch of the sweep pass from FETCH_SIZE and WRITE_SIZE
+    (KiB per dispatch), each scaled by what the same counter reads for a
+    256 MiB torch copy (256 MiB read, 256 MiB written)."""
+    probe = ["--traffic-probe", "--side", str(side)]
+    copy = ["--copy-probe", "256"]
+    f_sweep = counter_per_dispatch(probe, "FETCH_SIZE", "sweep_plain2d")
+    w_sweep = counter_per_dispatch(probe, "WRITE_SIZE", "sweep_plain2d")
+    f_copy = counter_per_dispatch(copy, "FETCH_SIZE", "copyBuffer")
+    w_copy = counter_per_dispatch(copy, "WRITE_SIZE", "copyBuffer")
+    if None in (f_sweep, w_sweep, f_copy, w_copy) or not f_copy or not w_copy:
+        return None
+    true_kib = 256 * 1024
+    f_scale, w_scale = true_kib / f_copy, true_kib / w_copy
+    return {"bytes": (f_sweep * f_scale + w_sweep * w_scale) * 1024.0,
+            "read_bytes": f_sweep * f_scale * 1024.0, "write_bytes": w_sweep * w_scale * 1024.0,
+            "fetch_scale": f_scale, "write_scale": w_scale}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -238,6 +287,10 @@ def main() -> int:
     ap.add_argument("--ensemble-mcs", type=int, default=100)
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="skip the rocprofv3 counter runs that measure the pass's HBM bytes")
+    ap.add_argument("--traffic-probe", action="store_true",
+                    help="only run a few passes of one lattice (the counter runs use this)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy a MIB-mebibyte buffer 5 times with torch, so a counter "
                          "run can scale FETCH_SIZE/WRITE_SIZE by a copy of known size")
@@ -262,6 +315,13 @@ def main() -> int:
     from glazier_amd.workloads import plain_monolayer
 
     side = args.side
+    if args.traffic_probe:
+        model, labels, types = plain_monolayer(side, seed=1, surface=True)
+        sim = HipSimulation(model, labels, types)
+        sim.step(4)
+        torch.cuda.synchronize()
+        sim.close()
+        return 0
     sims = []
     for seed in (1, 2, 3, 4):
         model, labels, types = plain_monolayer(side, seed=seed, surface=True)
@@ -320,12 +380,17 @@ def main() -> int:
         "hbm_frac_algorithmic": attempts * (16.0 + 4.0 * a) / elapsed / (HBM_PEAK_GBS * 1e9),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "sweep_plain2d_kernel (one colour pass)",
+                     "kernel": "sweep_plain2d_kernel (one colour pass)", "traffic_unit": "bytes per launch",
                      "avg_launch_us": avg_s * 1e6, "algorithmic_bytes_per_launch": bytes_per_launch},
         "cpu_baseline": None,
     }
     for s in sims:
         s.close()
+    if not args.no_traffic:
+        traffic = measured_traffic(side)
+        if traffic is not None:
+            line["roofline"]["traffic"] = traffic["bytes"]
+            line["roofline"]["traffic_detail"] = traffic
     if not args.no_ensemble:
         line["ensemble"] = ensemble_bench(args.ensemble_mcs)
     leg = None
