@@ -15,7 +15,10 @@ The same line carries "ensemble": the batched-versus-looped comparison on
 the uneven ensemble (64 infection sheets of 64^2..512^2 plus 16 volumes of
 64^3, parameters drawn by seed 3), each lattice run for --ensemble-mcs steps
 once as one batch and once lattice by lattice; replicas/s is lattices
-finished per second of wall time, setup excluded. And "adjoint": the
+finished per second of wall time, setup excluded. "configs" has the other
+workloads (config 0, config 1 at 1024^2 and 2048^2, config 2, and the
+reference's own benchmark model beside its published device rates). And
+"adjoint": the
 gradient of the 256^2 field fit (forward and backward times, agreement with
 the f64 CPU adjoint, and 50 Adam steps on the perturbed rates).
 """
@@ -205,6 +208,24 @@ def adjoint_bench(cpu: bool, adam_steps: int = 50):
     return line
 
 
+def cpu_baseline_configs():
+    """The serial oracle (one thread) on config 0 in full and on a bounded
+    sample of config 2 (the first 10 of its 200 steps)."""
+    import oracle
+    from glazier_amd.workloads import infection_sheet, plain_monolayer
+
+    out = {}
+    for name, build, steps in (("config0_128", plain_monolayer(128, 1, False), 100),
+                               ("config2_infection_512", infection_sheet(512, 7), 10)):
+        sim = oracle.SerialSimulation(*build)
+        t0 = time.perf_counter()
+        sim.step(steps)
+        el = time.perf_counter() - t0
+        out[name] = {"value": build[0].sites * steps / el, "unit": "site-attempts/s", "cores": 1,
+                     "kind": "port", "sample": f"{steps} steps in {el:.2f} s"}
+    return out
+
+
 def cpu_baseline_adjoint(leg: dict):
     """The f64 CPU adjoint (the oracle, one thread) of the same 256^2 fit:
     its time for one gradient, and how far the device gradient sits from it."""
@@ -225,6 +246,64 @@ def cpu_baseline_adjoint(leg: dict):
             "sample": "f64 replay and adjoint oracle, the whole 256^2 200-step fit, one thread",
             "device_vs_f64_max_rel": float(max(abs(gv[i] - wv[i]) / max(abs(wv[i]), 1e-3 * scale)
                                                for i in range(gv.size)))}
+
+
+def configs_bench():
+    """The other workloads, each from resident inputs: config 0 (128^2 plain,
+    100 steps, seed 1), config 1 at 1024^2 and 2048^2 (100 steps, seeds
+    1-4), config 2 (512^2 infection sheet, 200 steps, seed 7), and the
+    reference's own benchmark model (examples/bench.rs:7-25: contact 16/8,
+    V_t 64, lambda_V 2, T 10, Moore, seed 1, 100 steps) at the sizes whose
+    device times the reference publishes."""
+    import torch
+
+    from glazier_amd import HipSimulation
+    from glazier_amd.model import CellType, Model, default_cell_types, tiled_labels
+    from glazier_amd.workloads import infection_sheet, plain_monolayer
+
+    def run(build, steps, bytes_per_attempt):
+        sims = [HipSimulation(*b) for b in build]
+        for s in sims:
+            s.step(1)
+        torch.cuda.synchronize()
+        acc0 = sum(s.counters()[1] for s in sims)
+        t0 = time.perf_counter()
+        for s in sims:
+            s.step(steps)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        sites = sims[0].sites
+        attempts = sites * steps * len(sims)
+        a = (sum(s.counters()[1] for s in sims) - acc0) / attempts
+        for s in sims:
+            s.close()
+        return {"site_attempts_per_s": attempts / el, "mcs_per_s": steps * len(sims) / el,
+                "acceptance": a, "seconds": el,
+                "hbm_frac_algorithmic": attempts * bytes_per_attempt(a) / el / (HBM_PEAK_GBS * 1e9)}
+
+    def reference_model(side):
+        c = CellType(target_volume=64.0, lambda_volume=2.0)
+        m = Model(width=side, height=side, contact=[0.0, 16.0, 16.0, 8.0], types=[CellType(), c],
+                  temperature=10.0, neighbour_order=2, seed=1)
+        n = side // 8
+        return m, tiled_labels(m, 8, n, n, 1), default_cell_types(n * n + 1)
+
+    plain = lambda a: 16.0 + 4.0 * a
+    # infection: 4 colours x (labels + the chemotactic field) + accepted
+    # writes, then per site per step two fields read and written, the owner
+    # read of the exchange, and the moments' label read
+    infection = lambda a: 4.0 * (4.0 + 4.0) + 4.0 * a + 8.0 * 2 + 4.0 + 4.0
+    out = {"config0_128": run([plain_monolayer(128, 1, False)], 100, plain)}
+    for side in (1024, 2048):
+        out[f"config1_{side}"] = run([plain_monolayer(side, seed, True) for seed in (1, 2, 3, 4)], 100, plain)
+    out["config2_infection_512"] = run([infection_sheet(512, 7)], 200, infection)
+    published = {128: 8.2e8, 1024: 4.4e9, 4096: 3.76e9}
+    for side, ref in published.items():
+        r = run([reference_model(side)], 100, plain)
+        r["reference_published_site_attempts_per_s"] = ref
+        r["ratio_to_reference"] = r["site_attempts_per_s"] / ref
+        out[f"reference_model_{side}"] = r
+    return out
 
 
 def counter_per_dispatch(args, counter: str, kernel: str):
@@ -287,6 +366,7 @@ def main() -> int:
     ap.add_argument("--ensemble-mcs", type=int, default=100)
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-configs", action="store_true")
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the rocprofv3 counter runs that measure the pass's HBM bytes")
     ap.add_argument("--traffic-probe", action="store_true",
@@ -391,6 +471,8 @@ def main() -> int:
         if traffic is not None:
             line["roofline"]["traffic"] = traffic["bytes"]
             line["roofline"]["traffic_detail"] = traffic
+    if not args.no_configs:
+        line["configs"] = configs_bench()
     if not args.no_ensemble:
         line["ensemble"] = ensemble_bench(args.ensemble_mcs)
     leg = None
@@ -401,6 +483,8 @@ def main() -> int:
         line["cpu_baseline"] = cpu_baseline(side, args.cpu_seconds)
         if leg is not None:
             line["cpu_baseline"]["adjoint"] = cpu_baseline_adjoint(leg)
+        if not args.no_configs:
+            line["cpu_baseline"]["configs"] = cpu_baseline_configs()
     print(json.dumps(line))
     return 0
 
