@@ -100,6 +100,7 @@ enum BatchCellOp : int {
 hipError_t launch_batch_prepare_sweep(const BatchArgs &b, int phase, hipStream_t s);
 hipError_t launch_batch_sweep(const BatchArgs &b, bool d3, bool full, bool moore, const WorkList &w,
                               int threads, size_t lds, hipStream_t s);
+hipError_t launch_batch_length(const BatchArgs &b, const WorkList &w, hipStream_t s);
 hipError_t launch_batch_prepare_field(const BatchArgs &b, int chunk, hipStream_t s);
 hipError_t launch_batch_field(const BatchArgs &b, bool d3, const WorkList &w, size_t lds, hipStream_t s);
 hipError_t launch_batch_prepare_cells(const BatchArgs &b, hipStream_t s);

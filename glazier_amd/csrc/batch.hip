@@ -87,6 +87,18 @@ __global__ __launch_bounds__(512) void batch_sweep_kernel(BatchArgs b, WorkList 
     sweep_body<D3, FULL, MOORE2D>(a, (int)(blockIdx.x - w.first[g]));
 }
 
+// The length term's per-label `before` of this phase's pass, for every
+// replica whose model prices length (launched after the phase's prepare).
+__global__ __launch_bounds__(256) void batch_length_kernel(BatchArgs b, WorkList w) {
+    const uint32_t g = find_entry(w, blockIdx.x);
+    const uint32_t r = w.members[g];
+    const SweepArgs &s = b.cur_sweep[r];
+    if (s.n_off == 0) return;
+    length_prepass_body(s.a_cur, s.e_prev, s.n_labels, s.cell_words, b.reps[r].cell.ltype, s.tables, s.d > 1,
+                        const_cast<double *>(s.len_before), (int)(blockIdx.x - w.first[g]),
+                        (int)(w.first[g + 1] - w.first[g]));
+}
+
 // ---------------------------------------------------------------- fields
 
 // One thread per (replica, species): the arguments of field launch `chunk`
@@ -249,6 +261,12 @@ hipError_t launch_batch_sweep(const BatchArgs &b, bool d3, bool full, bool moore
         else GHIP_BS(false, false, false);
     }
 #undef GHIP_BS
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_length(const BatchArgs &b, const WorkList &w, hipStream_t s) {
+    if (w.blocks == 0) return hipSuccess;
+    hipLaunchKernelGGL(batch_length_kernel, dim3(w.blocks), dim3(256), 0, s, b, w);
     return hipGetLastError();
 }
 

@@ -136,6 +136,8 @@ struct ghip_batch {
     std::vector<SweepGroup> groups;
     std::vector<FieldLaunch> fields;
     std::vector<CellLaunch> cells;
+    ListRef length_ref;
+    WorkList length{};          // replicas that price length: the per-pass prepass
     int max_colours = 0, max_chunks = 0;
     bool any_cells = false, any_mot = false;
     std::vector<uint64_t> expect_mcs;  // each lattice's step count as the batch left it
@@ -305,6 +307,17 @@ ghip_status rebuild(ghip_batch *b) {
             fl.ref = add_list(b, members, blocks);
             b->fields.push_back(fl);
         }
+    // the length term's per-pass prepass
+    {
+        std::vector<uint32_t> members, blocks;
+        for (size_t r = 0; r < R; ++r) {
+            const ghip_sim *s = b->sims[r];
+            if (!(s->sweep_flags & kLength)) continue;
+            members.push_back((uint32_t)r);
+            blocks.push_back(cdiv(s->cap, 256));
+        }
+        b->length_ref = add_list(b, members, blocks);
+    }
     // per-label launches and the activity decay
     b->cells.clear();
     for (int op = 0; op < kNumCellOps; ++op) {
@@ -334,6 +347,7 @@ ghip_status rebuild(ghip_batch *b) {
     for (auto &g : b->groups) g.w = materialise(b, g.ref);
     for (auto &f : b->fields) f.w = materialise(b, f.ref);
     for (auto &c : b->cells) c.w = materialise(b, c.ref);
+    b->length = materialise(b, b->length_ref);
     HIP_TRY(hipMemcpy(b->d_reps, b->reps.data(), R * sizeof(BatchReplica), hipMemcpyHostToDevice));
     // the per-launch records start as the static arguments; the prepare
     // kernels write only what changes from pass to pass
@@ -370,6 +384,7 @@ ghip_status enqueue_step(ghip_batch *b, std::vector<std::pair<int, std::pair<hip
     };
     for (int ph = 0; ph < b->max_colours; ++ph) {
         HIP_TRY(launch_batch_prepare_sweep(A, ph, s));
+        HIP_TRY(launch_batch_length(A, b->length, s));
         for (size_t g = 0; g < b->groups.size(); ++g) {
             const SweepGroup &G = b->groups[g];
             if (ph >= G.colours) continue;

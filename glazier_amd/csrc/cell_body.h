@@ -412,6 +412,22 @@ __device__ __forceinline__ void compact_finish_body(const CellArgs &a, const int
     if (lb == 0 && threadIdx.x == 0) *a.n_labels = a.flags[3];
 }
 
+// Length minus target of every label in the snapshot a + e: the length
+// term's `before` (sweep_body.h), once per label instead of per attempt.
+__device__ __forceinline__ void length_prepass_body(const int64_t *a, const int64_t *e, const uint32_t *n_labels,
+                                                    int cw, const uint32_t *ltype, const DeviceTables *tables,
+                                                    bool d3, double *out, const int lb, const int gb) {
+    const uint32_t l = lb * blockDim.x + threadIdx.x;
+    if (l == 0 || l >= *n_labels) return;
+    const size_t base = (size_t)l * cw;
+    int64_t m[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) m[q] = a[base + 1 + q] + e[base + 1 + q];
+    const int64_t n = unpack_vol(a[base] + e[base]);
+    const uint32_t ty = label_type(ltype[l]);
+    out[l] = length_of(m, n, d3) - tables->tgt_len[ty];
+}
+
 __device__ __forceinline__ void copy_u32_body(uint32_t *dst, const uint32_t *src, const uint32_t *n_labels, int per, const int lb, const int gb) {
     size_t n = (size_t)(*n_labels) * per;
     for (size_t i = (size_t)lb * blockDim.x + threadIdx.x; i < n;

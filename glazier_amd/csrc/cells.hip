@@ -81,9 +81,22 @@ __global__ void copy_u32_kernel(uint32_t *dst, const uint32_t *src, const uint32
     copy_u32_body(dst, src, n_labels, per, blockIdx.x, gridDim.x);
 }
 
+__global__ void length_prepass_kernel(const int64_t *a, const int64_t *e, const uint32_t *n_labels, int cw,
+                                      const uint32_t *ltype, const DeviceTables *tables, int d3, double *out) {
+    length_prepass_body(a, e, n_labels, cw, ltype, tables, d3 != 0, out, blockIdx.x, gridDim.x);
+}
+
 // ---------------------------------------------------------------- launchers
 
 static int blocks_for(size_t n, int threads) { return (int)((n + threads - 1) / threads); }
+
+hipError_t launch_length_prepass(const int64_t *a, const int64_t *e, const uint32_t *n_labels, int cw,
+                                 const uint32_t *ltype, const DeviceTables *tables, bool d3, double *out,
+                                 uint32_t capacity, hipStream_t s) {
+    hipLaunchKernelGGL(length_prepass_kernel, dim3(blocks_for(capacity, 256)), dim3(256), 0, s, a, e, n_labels,
+                       cw, ltype, tables, d3 ? 1 : 0, out);
+    return hipGetLastError();
+}
 
 hipError_t launch_fill_i64(int64_t *p, size_t n, hipStream_t s) {
     if (n == 0) return hipSuccess;

@@ -253,6 +253,9 @@ ghip_status run_sweep_pass(ghip_sim *s, int colour) {
     a.xcd_chunk = (s->blocks % 8 == 0) ? s->blocks / 8 : 0;
     TimedLaunch t{};
     HIP_TRY(s->begin_timed(0, &t));
+    if (s->sweep_flags & kLength)
+        HIP_TRY(launch_length_prepass(a.a_cur, a.e_prev, s->n_labels_dev, s->cw, s->ltype, s->tables, s->d3,
+                                      s->len_before, s->cap, s->stream));
     if (s->plain_fast) {
         PlainParams pp{};
         for (int x = 0; x < 2; ++x)
@@ -484,6 +487,7 @@ ghip_status ghip_create(const ghip_model *model, const uint32_t *labels, size_t 
     CREATE_TRY(hipMalloc(&s->daughter, (size_t)cap * 4));
     CREATE_TRY(hipMalloc(&s->plan, (size_t)cap));
     CREATE_TRY(hipMalloc(&s->geo, (size_t)cap * 6 * 8));
+    if (s->sweep_flags & kLength) CREATE_TRY(hipMalloc(&s->len_before, (size_t)cap * 8));
     CREATE_TRY(hipMalloc(&s->flags, 16));
     CREATE_TRY(hipMalloc(&s->type_copy, (size_t)cap * 4));
     CREATE_TRY(hipMalloc(&s->anchor_copy, (size_t)cap * 3 * 4));

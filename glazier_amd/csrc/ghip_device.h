@@ -62,6 +62,7 @@ struct SweepArgs {
     int64_t *a_next;             // base of the next pass: a_cur + e_prev
     int64_t *g_zero;             // delta buffer of the pass after next, cleared
     const int32_t *anchor;       // 3 per label
+    const double *len_before;    // per label: length minus target in this pass's snapshot
     const DeviceTables *tables;
     const uint32_t *n_labels;    // device label count
     unsigned long long *accepted; // kAcceptSlots counters
@@ -139,6 +140,11 @@ void plain2d_tile(int w, int h, int *px, int *py);
 hipError_t launch_field(const FieldArgs &a, bool d3, int blocks, hipStream_t stream);
 size_t field_lds_bytes(const FieldArgs &a, bool d3);
 hipError_t launch_activity_decay(float *act, size_t n, hipStream_t stream);
+// Each label's length minus its type's target length in the snapshot
+// a + e (the length term's `before`, the same for every attempt of a pass).
+hipError_t launch_length_prepass(const int64_t *a, const int64_t *e, const uint32_t *n_labels, int cw,
+                                 const uint32_t *ltype, const DeviceTables *tables, bool d3, double *out,
+                                 uint32_t capacity, hipStream_t stream);
 hipError_t launch_fill_i64(int64_t *p, size_t n, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t *p, size_t n, uint32_t v, hipStream_t s);
 hipError_t launch_consolidate(int64_t *a, int64_t *e, const uint32_t *n_labels, int cw, size_t cap,

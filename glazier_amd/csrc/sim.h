@@ -65,6 +65,7 @@ struct ghip_sim {
     uint32_t *daughter = nullptr;
     uint8_t *plan = nullptr;
     double *geo = nullptr;
+    double *len_before = nullptr;  // per label, when the length term is on
     uint32_t *flags = nullptr;
     uint32_t *type_copy = nullptr;
     int32_t *anchor_copy = nullptr;
@@ -104,7 +105,7 @@ struct ghip_sim {
         if (stream) (void)hipStreamSynchronize(stream);
         void *bufs[] = {labels[0], labels[1], act[0], act[1], field_base[0], field_base[1], A[0],
                         A[1], Dl[0], Dl[1], Dl[2], anchor, ltype, scratch, daughter, plan, geo,
-                        flags, type_copy, anchor_copy, n_labels_dev, tables, accepted, traj};
+                        flags, type_copy, anchor_copy, n_labels_dev, tables, accepted, traj, len_before};
         for (void *p : bufs)
             if (p) (void)hipFree(p);
         for (auto *list : {&pending, &spare})
@@ -147,6 +148,7 @@ struct ghip_sim {
         a.act = act[act_cur];
         a.act_prev = a.act;
         a.anchor = anchor;
+        a.len_before = len_before;
         a.tables = tables;
         a.n_labels = n_labels_dev;
         a.accepted = accepted;

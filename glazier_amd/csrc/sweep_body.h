@@ -298,7 +298,7 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &a, const int blk) {
                     after[6] = m[6] + sign * ux * uy;
                     after[7] = m[7] + sign * ux * uz;
                     after[8] = m[8] + sign * uy * uz;
-                    double before = length_of(m, n, three) - tab->tgt_len[ty];
+                    double before = a.len_before[id];
                     double now = length_of(after, n + sign, three) - tab->tgt_len[ty];
                     dh = dh + (float)(tab->lam_len[ty] * (now * now - before * before));
                 }
