@@ -199,9 +199,9 @@ void choose_tiles(ghip_sim *s) {
     s->blocks = s->tiles_x * s->tiles_y * s->tiles_z;
     int px = 0, py = 0;
     plain2d_tile(s->W, s->H, &px, &py);
-    s->plain_fast = !s->d3 && s->n_off == 8 && s->m.n_types == 2 &&
-                    (s->sweep_flags & kIntegral) && !s->full && px > 0 &&
-                    plain2d_supported(s->host_tables.n_thr);
+    s->plain_physics = !s->d3 && s->n_off == 8 && s->m.n_types == 2 && (s->sweep_flags & kIntegral) &&
+                       !s->full && plain2d_supported(s->host_tables.n_thr);
+    s->plain_fast = s->plain_physics && px > 0;
     if (s->plain_fast) {
         s->tiles_x = s->W / px;
         s->tiles_y = s->H / py;
@@ -225,6 +225,60 @@ void choose_tiles(ghip_sim *s) {
     s->f_tiles_y = (s->H + (1 << s->fy_log2) - 1) >> s->fy_log2;
     int f_tiles_z = s->d3 ? (s->D + s->fz - 1) / s->fz : 1;
     s->f_blocks = s->f_tiles_x * s->f_tiles_y * f_tiles_z;
+}
+
+PlainParams plain_params(const ghip_sim *s) {
+    PlainParams pp{};
+    for (int x = 0; x < 2; ++x)
+        for (int y = 0; y < 2; ++y) pp.j[x][y] = s->host_tables.J[x][y];
+    pp.lam_v = s->host_tables.lam_v[1];
+    pp.tgt_v = s->host_tables.tgt_v[1];
+    pp.lam_s = s->host_tables.lam_s[1];
+    pp.tgt_s = s->host_tables.tgt_s[1];
+    pp.thr = s->tables->thr;
+    pp.n_thr = s->host_tables.n_thr;
+    // with both contact rows flat across the medium and the cell type, a
+    // medium neighbour prices like a cell one and the medium count drops out
+    pp.needs_medium = !(pp.j[0][0] == pp.j[0][1] && pp.j[1][0] == pp.j[1][1]);
+    return pp;
+}
+
+// Steps a small plain lattice inside one workgroup, at most kResidentChunk
+// steps per launch so no launch runs long. Pass and step counters advance as
+// the per-pass driver's would; the record rotation lands where it would.
+constexpr uint64_t kResidentChunk = 64;
+
+ghip_status run_resident(ghip_sim *s, uint64_t steps) {
+    const PlainParams pp = plain_params(s);
+    while (steps > 0) {
+        const uint32_t take = (uint32_t)std::min<uint64_t>(steps, kResidentChunk);
+        const uint64_t p0 = s->pass, p1 = p0 + 4 * (uint64_t)take;
+        PlainResidentArgs r{};
+        r.labels = s->labels[s->lab_cur];
+        r.a_start = s->A[p0 % 2];
+        r.e_start = s->Dl[(p0 + 2) % 3];
+        r.a_end = s->A[p1 % 2];
+        r.e_clear0 = s->Dl[(p1 + 2) % 3];
+        r.e_clear1 = s->Dl[p1 % 3];
+        r.n_labels = s->n_labels_dev;
+        r.accepted = s->accepted;
+        r.seed = s->m.seed;
+        r.mcs0 = s->mcs;
+        r.replica = s->m.replica;
+        r.steps = take;
+        r.w_log2 = ilog2(s->W);
+        r.h = s->H;
+        r.capacity = s->cap;
+        TimedLaunch t{};
+        HIP_TRY(s->begin_timed(0, &t));
+        HIP_TRY(launch_sweep_plain2d_resident(r, pp, s->stream));
+        HIP_TRY(s->end_timed(t));
+        s->pass = p1;
+        s->mcs += take;
+        s->attempted += (uint64_t)s->N * take;
+        steps -= take;
+    }
+    return GHIP_OK;
 }
 
 ghip_status run_sweep_pass(ghip_sim *s, int colour) {
@@ -257,19 +311,7 @@ ghip_status run_sweep_pass(ghip_sim *s, int colour) {
         HIP_TRY(launch_length_prepass(a.a_cur, a.e_prev, s->n_labels_dev, s->cw, s->ltype, s->tables, s->d3,
                                       s->len_before, s->cap, s->stream));
     if (s->plain_fast) {
-        PlainParams pp{};
-        for (int x = 0; x < 2; ++x)
-            for (int y = 0; y < 2; ++y) pp.j[x][y] = s->host_tables.J[x][y];
-        pp.lam_v = s->host_tables.lam_v[1];
-        pp.tgt_v = s->host_tables.tgt_v[1];
-        pp.lam_s = s->host_tables.lam_s[1];
-        pp.tgt_s = s->host_tables.tgt_s[1];
-        pp.thr = s->tables->thr;
-        pp.n_thr = s->host_tables.n_thr;
-        // with both contact rows flat across the medium and the cell type, a
-        // medium neighbour prices like a cell one and the medium count drops out
-        pp.needs_medium = !(pp.j[0][0] == pp.j[0][1] && pp.j[1][0] == pp.j[1][1]);
-        HIP_TRY(launch_sweep_plain2d(a, pp, s->blocks, s->stream));
+        HIP_TRY(launch_sweep_plain2d(a, plain_params(s), s->blocks, s->stream));
     } else {
         HIP_TRY(launch_sweep(a, s->d3, s->full, s->blocks, s->threads, s->stream));
     }
@@ -447,6 +489,7 @@ ghip_status ghip_create(const ghip_model *model, const uint32_t *labels, size_t 
     if (s->has_pop) cap = std::max<uint64_t>(2 * n_labels + 1024, n_labels + N / 16);
     cap = std::min<uint64_t>(cap, kIdMask - 1);
     s->cap = (uint32_t)cap;
+    s->resident = s->plain_physics && m.n_species == 0 && plain2d_resident_fits(s->W, s->H, s->cap);
     for (uint32_t k = 0; k < m.n_species; ++k) {
         s->substeps[k] = substeps_for(m, k);
         s->fd[k] = (float)(m.species[k].diffusion / (double)s->substeps[k]);
@@ -549,6 +592,11 @@ ghip_status ghip_step(ghip_sim *s, uint64_t steps) {
     if (s->traj && s->traj_len + steps > s->traj_cap)
         return fail(GHIP_BAD_ARG, "the recording has room for " + std::to_string(s->traj_cap - s->traj_len) +
                                       " more steps");
+    if (s->resident && !s->traj) {
+        ghip_status st = run_resident(s, steps);
+        if (st != GHIP_OK) return st;
+        steps = 0;
+    }
     for (uint64_t q = 0; q < steps; ++q) {
         int order[8];
         colour_order(s->m.seed, s->m.replica, s->mcs, s->colours, order);

@@ -92,6 +92,23 @@ struct PlainParams {
     uint32_t needs_medium;  // whether a medium neighbour can change a price
 };
 
+// A small plain lattice stepped whole in one workgroup (sweep_plain.hip):
+// `steps` Monte Carlo steps from step `mcs0`, the per-label records read as
+// a + e at the first pass and left as a_end (with both delta buffers the
+// next pass touches cleared) after the last.
+struct PlainResidentArgs {
+    uint32_t *labels;             // w x h packed words, read and written back
+    const int64_t *a_start, *e_start;
+    int64_t *a_end;
+    int64_t *e_clear0, *e_clear1;
+    const uint32_t *n_labels;
+    unsigned long long *accepted; // kAcceptSlots counters
+    uint64_t seed, mcs0;
+    uint32_t replica, steps;
+    int w_log2, h;
+    uint32_t capacity;            // label records the lattice may name
+};
+
 // One launch of the field step: `steps` substeps of one species.
 struct FieldArgs {
     const float *in;        // one species, n sites
@@ -137,6 +154,10 @@ hipError_t launch_sweep_plain2d(const SweepArgs &a, const PlainParams &p, int bl
 bool plain2d_supported(uint32_t n_thr);
 // Tile of the plain pass for a w x h lattice (0 x 0: the pass does not apply).
 void plain2d_tile(int w, int h, int *px, int *py);
+// Whether a w x h lattice with `capacity` label records fits the resident pass.
+bool plain2d_resident_fits(int w, int h, uint32_t capacity);
+hipError_t launch_sweep_plain2d_resident(const PlainResidentArgs &r, const PlainParams &p,
+                                         hipStream_t stream);
 hipError_t launch_field(const FieldArgs &a, bool d3, int blocks, hipStream_t stream);
 size_t field_lds_bytes(const FieldArgs &a, bool d3);
 hipError_t launch_activity_decay(float *act, size_t n, hipStream_t stream);

@@ -86,7 +86,9 @@ struct ghip_sim {
 
     int tx_log2 = 6, ty_log2 = 3, tz_log2 = 0, threads = 512;
     int tiles_x = 1, tiles_y = 1, tiles_z = 1, blocks = 1;
-    bool plain_fast = false;  // the plain-monolayer pass (sweep_plain.hip) applies
+    bool plain_physics = false;  // the medium, one cell type, integer contact/volume/surface, Moore plane
+    bool plain_fast = false;     // the plain-monolayer pass (sweep_plain.hip) applies
+    bool resident = false;       // the whole lattice steps inside one workgroup
     int fx_log2 = 6, fy_log2 = 5, fz = 1, f_tiles_x = 1, f_tiles_y = 1, f_blocks = 1, f_kmax = 4;
 
     bool timing = false;

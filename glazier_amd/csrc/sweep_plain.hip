@@ -5,12 +5,16 @@
 // Same draws, same decisions and the same bookkeeping as sweep_kernel (and so
 // as the CPU emulation); only the work is organised for gfx950:
 //
-//  * a workgroup owns a PX x PY tile (256 x 32 sites with 512 lanes where the
-//    lattice divides, else 128 x 16 with 256 lanes). Every global read of the
-//    workgroup (the PY + 2 tile rows as 16-byte loads, the halo columns, the
-//    acceptance table and its share of the snapshot fold) is issued before
-//    the first wait; rows land in an LDS tile padded to a PX + 8 word pitch
-//    (interior at column 4, so each 16-byte LDS write is aligned);
+//  * a workgroup owns a PX x PY tile (256 x 32 sites with 512 lanes above
+//    512^2 where the lattice divides, else 128 x 16 with 256 lanes). Every
+//    global read of the workgroup (the PY + 2 tile rows as 16-byte loads,
+//    the halo columns, the acceptance table and its share of the snapshot
+//    fold) is issued before the first wait; rows land in an LDS tile of pitch PX + 4 words, the
+//    interior at column 4 (so each 16-byte LDS write is aligned), the left
+//    halo at column 3 and the right halo one past the interior, which is
+//    column 0 of the next row. At 256 x 32 that keeps the workgroup under
+//    40 KiB of LDS, four per CU, and 2048 tiles of a 4096^2 lattice fill
+//    the chip in exactly two rounds;
 //  * phase A draws every target. A wavefront's 64 targets share one tile row,
 //    so row bases stay in scalar registers; the target's label and its picked
 //    neighbour come from LDS, and the targets whose label would change are
@@ -40,11 +44,75 @@ __device__ __forceinline__ uint32_t unequal8(uint32_t v, uint32_t n0, uint32_t n
     const uint32_t b = min(n3 ^ v, 1u) + min(n4 ^ v, 1u) + min(n5 ^ v, 1u);
     return a + b + min(n6 ^ v, 1u) + min(n7 ^ v, 1u);
 }
+
+// The decision on one target whose label would change from `old` to `nw`:
+// contact gained minus lost from neighbour counts (exact with integer
+// energies), volume and surface against each label's pass-start record
+// (volume * 2^32 + surface), then the integral acceptance table. Shared by
+// the streaming pass and the resident one, so both decide identically.
+struct PlainMove {
+    bool acc;
+    int ds_old, ds_new;
+};
+
+__device__ __forceinline__ PlainMove plain_move(const PlainParams &p, const uint32_t *thr_s, uint32_t r24,
+                                                uint32_t old, uint32_t nw, uint32_t n0, uint32_t n1,
+                                                uint32_t n2, uint32_t n3, uint32_t n4, uint32_t n5,
+                                                uint32_t n6, uint32_t n7, int64_t pv_old, int64_t pv_new,
+                                                bool has_medium) {
+    const uint32_t oid = label_id(old), nid = label_id(nw);
+    const int unlike_old = (int)unequal8(old, n0, n1, n2, n3, n4, n5, n6, n7);
+    const int unlike_new = (int)unequal8(nw, n0, n1, n2, n3, n4, n5, n6, n7);
+    const int like_old = 8 - unlike_old, like_new = 8 - unlike_new;
+    // absolute states keep surface in [0, 2^31): volume is the high word
+    const int vol_old = (int)(uint32_t)((uint64_t)pv_old >> 32);
+    const int vol_new = (int)(uint32_t)((uint64_t)pv_new >> 32);
+    // a copy that would empty a cell is refused (no branch before the
+    // records are used, so no gather waits behind another)
+    const bool emptying = oid != 0 && vol_old <= 1;
+    int med = 0;
+    if (has_medium) med = 8 - (int)unequal8(0u, n0, n1, n2, n3, n4, n5, n6, n7);
+    const int etn = nid != 0 ? 1 : 0, eto = oid != 0 ? 1 : 0;
+    // neighbours unlike the new label: 8 - like_new; the medium ones among
+    // them are all medium neighbours when the new label is a cell, none
+    // when it is the medium (then like_new counts them). So `med` enters
+    // only beside a cell label, where flat contact rows cancel it.
+    const int med_ne_new = nid != 0 ? med : 0;
+    const int cell_ne_new = unlike_new - med_ne_new;
+    const int med_ne_old = oid != 0 ? med : 0;
+    const int cell_ne_old = unlike_old - med_ne_old;
+    const float cn = p.j[etn][0] * (float)med_ne_new + p.j[etn][1] * (float)cell_ne_new;
+    const float co = p.j[eto][0] * (float)med_ne_old + p.j[eto][1] * (float)cell_ne_old;
+    float dh = cn - co;
+    PlainMove mv;
+    mv.ds_old = 2 * like_old - 8;
+    mv.ds_new = 8 - 2 * like_new;
+    if (oid != 0 && p.lam_v != 0.0f) {
+        const float vv = (float)vol_old;
+        dh = dh + p.lam_v * (1.0f - 2.0f * (vv - p.tgt_v));
+    }
+    if (nid != 0 && p.lam_v != 0.0f) {
+        const float vv = (float)vol_new;
+        dh = dh + p.lam_v * (1.0f + 2.0f * (vv - p.tgt_v));
+    }
+    if (oid != 0 && mv.ds_old != 0 && p.lam_s != 0.0f) {
+        const float s = (float)(int32_t)(uint32_t)pv_old;
+        const float dsf = (float)mv.ds_old;
+        dh = dh + p.lam_s * (dsf * (2.0f * (s - p.tgt_s) + dsf));
+    }
+    if (nid != 0 && mv.ds_new != 0 && p.lam_s != 0.0f) {
+        const float s = (float)(int32_t)(uint32_t)pv_new;
+        const float dsf = (float)mv.ds_new;
+        dh = dh + p.lam_s * (dsf * (2.0f * (s - p.tgt_s) + dsf));
+    }
+    mv.acc = !emptying && (dh <= 0.0f || (dh < (float)p.n_thr && r24 < thr_s[(int)dh]));
+    return mv;
+}
 }  // namespace
 
 template <int PX, int PY, int NT>
 __global__ __launch_bounds__(NT) void sweep_plain2d_kernel(SweepArgs a, PlainParams p) {
-    constexpr int kPitch = PX + 8;
+    constexpr int kPitch = PX + 4;
     constexpr int kRows = PY + 2;
     constexpr int kTargets = PX * PY / 4;
     constexpr int kC4 = PX / 4;                          // 16-byte loads per row
@@ -53,7 +121,7 @@ __global__ __launch_bounds__(NT) void sweep_plain2d_kernel(SweepArgs a, PlainPar
     static_assert(kTRow % 64 == 0, "a wavefront's targets share one row");
     static_assert(kTargets % NT == 0 && kTargets <= 65536, "targets per lane, 16-bit queue");
 
-    __shared__ __attribute__((aligned(16))) uint32_t tile[kRows * kPitch];
+    __shared__ __attribute__((aligned(16))) uint32_t tile[kRows * kPitch + 4];
     __shared__ uint16_t queue[kTargets];
     __shared__ uint32_t thr_s[kPThr];
     __shared__ uint32_t qcount;
@@ -196,52 +264,10 @@ __global__ __launch_bounds__(NT) void sweep_plain2d_kernel(SweepArgs a, PlainPar
         const uint32_t n0 = tile[tl - kPitch - 1], n1 = tile[tl - kPitch], n2 = tile[tl - kPitch + 1];
         const uint32_t n3 = tile[tl - 1], n4 = tile[tl + 1];
         const uint32_t n5 = tile[tl + kPitch - 1], n6 = tile[tl + kPitch], n7 = tile[tl + kPitch + 1];
-        const int unlike_old = (int)unequal8(old, n0, n1, n2, n3, n4, n5, n6, n7);
-        const int unlike_new = (int)unequal8(nw, n0, n1, n2, n3, n4, n5, n6, n7);
-        const int like_old = 8 - unlike_old, like_new = 8 - unlike_new;
-        const int64_t pv_old = ao + eo, pv_new = an + en;
-        // absolute states keep surface in [0, 2^31): volume is the high word
-        const int vol_old = (int)(uint32_t)((uint64_t)pv_old >> 32);
-        const int vol_new = (int)(uint32_t)((uint64_t)pv_new >> 32);
-        // a copy that would empty a cell is refused (no branch before the
-        // gathers are used, so none of them waits behind another)
-        const bool emptying = oid != 0 && vol_old <= 1;
-        int med = 0;
-        if (has_medium) med = 8 - (int)unequal8(0u, n0, n1, n2, n3, n4, n5, n6, n7);
-        // contact gained minus lost, from counts (exact: integer energies)
-        const int etn = nid != 0 ? 1 : 0, eto = oid != 0 ? 1 : 0;
-        // neighbours unlike the new label: 8 - like_new; the medium ones among
-        // them are all medium neighbours when the new label is a cell, none
-        // when it is the medium (then like_new counts them). So `med` enters
-        // only beside a cell label, where flat contact rows cancel it.
-        const int med_ne_new = nid != 0 ? med : 0;
-        const int cell_ne_new = unlike_new - med_ne_new;
-        const int med_ne_old = oid != 0 ? med : 0;
-        const int cell_ne_old = unlike_old - med_ne_old;
-        const float cn = p.j[etn][0] * (float)med_ne_new + p.j[etn][1] * (float)cell_ne_new;
-        const float co = p.j[eto][0] * (float)med_ne_old + p.j[eto][1] * (float)cell_ne_old;
-        float dh = cn - co;
-        const int ds_old = 2 * like_old - 8;
-        const int ds_new = 8 - 2 * like_new;
-        if (oid != 0 && p.lam_v != 0.0f) {
-            const float vv = (float)vol_old;
-            dh = dh + p.lam_v * (1.0f - 2.0f * (vv - p.tgt_v));
-        }
-        if (nid != 0 && p.lam_v != 0.0f) {
-            const float vv = (float)vol_new;
-            dh = dh + p.lam_v * (1.0f + 2.0f * (vv - p.tgt_v));
-        }
-        if (oid != 0 && ds_old != 0 && p.lam_s != 0.0f) {
-            const float s = (float)(int32_t)(uint32_t)pv_old;
-            const float dsf = (float)ds_old;
-            dh = dh + p.lam_s * (dsf * (2.0f * (s - p.tgt_s) + dsf));
-        }
-        if (nid != 0 && ds_new != 0 && p.lam_s != 0.0f) {
-            const float s = (float)(int32_t)(uint32_t)pv_new;
-            const float dsf = (float)ds_new;
-            dh = dh + p.lam_s * (dsf * (2.0f * (s - p.tgt_s) + dsf));
-        }
-        const bool acc = !emptying && (dh <= 0.0f || (dh < (float)p.n_thr && r24 < thr_s[(int)dh]));
+        const PlainMove mv =
+            plain_move(p, thr_s, r24, old, nw, n0, n1, n2, n3, n4, n5, n6, n7, ao + eo, an + en, has_medium);
+        const bool acc = mv.acc;
+        const int ds_old = mv.ds_old, ds_new = mv.ds_new;
         if (!acc) continue;
         a.labels[site] = nw;
         if (oid != 0)
@@ -258,10 +284,161 @@ __global__ __launch_bounds__(NT) void sweep_plain2d_kernel(SweepArgs a, PlainPar
         atomicAdd(a.accepted + (blockIdx.x & (kAcceptSlots - 1)), (unsigned long long)acc_block);
 }
 
+// ---- the resident pass: a small lattice stepped whole inside one workgroup
+//
+// Below about 128^2 sites a colour pass is a few microseconds of dependent
+// latency, and four launches a step cost more than the work. One workgroup
+// of 1024 lanes then holds the whole lattice and every label's record in
+// LDS and runs `steps` Monte Carlo steps, colour order and pass keys derived
+// in the kernel as the host derives them. Each pass keeps the per-colour
+// snapshot: the records stay fixed while the pass prices against them, its
+// copies accumulate into a second array, and a fold after the pass's last
+// barrier adds them in. The lattice is written straight into LDS: a target
+// only reads sites of the other colours, which no target of its pass writes.
+// On the way out the lattice, the folded records and cleared delta buffers
+// leave the global state exactly as the streaming passes would have.
+constexpr int kResThreads = 1024;
+constexpr int kResSites = 16384;   // lattice words held in LDS (64 KiB)
+constexpr int kResLabels = 2048;   // per-label records held in LDS, twice 16 KiB
+
+__global__ __launch_bounds__(kResThreads) void sweep_plain2d_resident_kernel(PlainResidentArgs r, PlainParams p) {
+    __shared__ __attribute__((aligned(16))) uint32_t lat[kResSites];
+    __shared__ int64_t rec[kResLabels];
+    __shared__ unsigned long long delta[kResLabels];
+    __shared__ uint16_t queue[kResSites / 4];
+    __shared__ uint32_t thr_s[kPThr];
+    __shared__ uint32_t qcount;
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wl = r.w_log2;
+    const int W = 1 << wl, H = r.h;
+    const uint32_t quads = ((uint32_t)H << wl) >> 2;  // 16-byte words of the lattice
+    const uint32_t nl = *r.n_labels;
+
+    for (uint32_t i = tid; i < quads; i += kResThreads)
+        reinterpret_cast<uint4 *>(lat)[i] = reinterpret_cast<const uint4 *>(r.labels)[i];
+    for (uint32_t i = tid; i < nl; i += kResThreads) {
+        rec[i] = r.a_start[i] + r.e_start[i];
+        delta[i] = 0ull;
+    }
+    if (tid < (int)p.n_thr) thr_s[tid] = p.thr[tid];
+    if (tid == 0) qcount = 0;
+    __syncthreads();
+
+    const int tw = wl - 1;                         // log2 of targets per row
+    const uint32_t nt = quads;                      // targets per pass: a quarter of the sites
+    const uint32_t nt_pad = (nt + kResThreads - 1) / kResThreads * kResThreads;
+    uint32_t my_acc = 0;
+    for (uint32_t step = 0; step < r.steps; ++step) {
+        const uint64_t mcs = r.mcs0 + step;
+        int order[4];
+        colour_order(r.seed, r.replica, mcs, 4, order);
+        for (int c = 0; c < 4; ++c) {
+            const int colour = order[c];
+            const int cx = colour & 1, cy = (colour >> 1) & 1;
+            const uint32_t key = pass_key(r.seed, r.replica, mcs, (uint32_t)colour);
+            const uint32_t key2 = second_key(key);
+
+            // phase A: every target's draw; queue the ones whose label would change
+            for (uint32_t q = tid; q < nt_pad; q += kResThreads) {
+                bool moves = false;
+                uint32_t site = 0;
+                if (q < nt) {
+                    const int x = 2 * (int)(q & ((1u << tw) - 1u)) + cx;
+                    const int y = 2 * (int)(q >> tw) + cy;
+                    site = ((uint32_t)y << wl) | (uint32_t)x;
+                    const uint32_t h = site_draw(key, key2, site);
+                    const uint32_t sh = (h & 7u) * 2u;
+                    const int nx = (x + (int)((0x9224u >> sh) & 3u) - 1) & (W - 1);
+                    int ny = y + (int)((0xA940u >> sh) & 3u) - 1;
+                    ny += ny < 0 ? H : 0;
+                    ny -= ny >= H ? H : 0;
+                    moves = lat[site] != lat[((uint32_t)ny << wl) | (uint32_t)nx];
+                }
+                const unsigned long long m = __ballot(moves);
+                if (m == 0) continue;
+                uint32_t base = 0;
+                if (lane == 0) base = atomicAdd(&qcount, (uint32_t)__builtin_popcountll(m));
+                base = __builtin_amdgcn_readfirstlane(base);
+                if (moves) {
+                    const uint32_t pos = base + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                                                          __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+                    queue[pos] = (uint16_t)site;
+                }
+            }
+            __syncthreads();
+
+            // phase B: price the queued targets against the pass-start records
+            const uint32_t nq = qcount;
+            for (uint32_t qi = tid; qi < nq; qi += kResThreads) {
+                const uint32_t site = queue[qi];
+                const int x = (int)(site & (uint32_t)(W - 1)), y = (int)(site >> wl);
+                const uint32_t h = site_draw(key, key2, site);
+                const uint32_t xm = (uint32_t)((x - 1) & (W - 1)), xp = (uint32_t)((x + 1) & (W - 1));
+                const uint32_t ym = (uint32_t)(y == 0 ? H - 1 : y - 1) << wl;
+                const uint32_t y0 = (uint32_t)y << wl;
+                const uint32_t yp = (uint32_t)(y == H - 1 ? 0 : y + 1) << wl;
+                const uint32_t n0 = lat[ym | xm], n1 = lat[ym | (uint32_t)x], n2 = lat[ym | xp];
+                const uint32_t n3 = lat[y0 | xm], n4 = lat[y0 | xp];
+                const uint32_t n5 = lat[yp | xm], n6 = lat[yp | (uint32_t)x], n7 = lat[yp | xp];
+                const uint32_t old = lat[site];
+                // the picked neighbour, k in ORDER3's planar order
+                const uint32_t k = h & 7u;
+                const uint32_t nw = k == 0 ? n0 : k == 1 ? n1 : k == 2 ? n2 : k == 3 ? n3
+                                  : k == 4 ? n4 : k == 5 ? n5 : k == 6 ? n6 : n7;
+                const uint32_t oid = label_id(old), nid = label_id(nw);
+                const PlainMove mv = plain_move(p, thr_s, h >> 8, old, nw, n0, n1, n2, n3, n4, n5, n6, n7,
+                                                rec[oid], rec[nid], p.needs_medium != 0);
+                if (!mv.acc) continue;
+                lat[site] = nw;
+                if (oid != 0) atomicAdd(&delta[oid], (unsigned long long)pack_vs(-1, mv.ds_old));
+                if (nid != 0) atomicAdd(&delta[nid], (unsigned long long)pack_vs(1, mv.ds_new));
+                ++my_acc;
+            }
+            __syncthreads();
+
+            // fold the pass's copies into the records the next pass reads
+            for (uint32_t i = tid; i < nl; i += kResThreads) {
+                rec[i] += (int64_t)delta[i];
+                delta[i] = 0ull;
+            }
+            if (tid == 0) qcount = 0;
+            __syncthreads();
+        }
+    }
+
+    for (uint32_t i = tid; i < quads; i += kResThreads)
+        reinterpret_cast<uint4 *>(r.labels)[i] = reinterpret_cast<const uint4 *>(lat)[i];
+    for (uint32_t i = tid; i < nl; i += kResThreads) {
+        r.a_end[i] = rec[i];
+        r.e_clear0[i] = 0;
+        r.e_clear1[i] = 0;
+    }
+    // per-lane counts summed across the wavefront, one atomic per wavefront
+    uint32_t t = my_acc;
+    for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off, 64);
+    if (lane == 0 && t) atomicAdd(r.accepted + ((tid >> 6) & (kAcceptSlots - 1)), (unsigned long long)t);
+}
+
+bool plain2d_resident_fits(int w, int h, uint32_t capacity) {
+    const bool pow2 = w >= 4 && (w & (w - 1)) == 0;
+    return pow2 && h >= 2 && h % 2 == 0 && (size_t)w * (size_t)h <= (size_t)kResSites &&
+           capacity <= (uint32_t)kResLabels;
+}
+
+hipError_t launch_sweep_plain2d_resident(const PlainResidentArgs &r, const PlainParams &p, hipStream_t stream) {
+    if (!plain2d_resident_fits(1 << r.w_log2, r.h, r.capacity)) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(sweep_plain2d_resident_kernel, dim3(1), dim3(kResThreads), 0, stream, r, p);
+    return hipGetLastError();
+}
+
 bool plain2d_supported(uint32_t n_thr) { return n_thr <= (uint32_t)kPThr; }
 
 void plain2d_tile(int w, int h, int *px, int *py) {
-    if (w % 256 == 0 && h % 32 == 0) {
+    // up to 512^2 a pass is a few tiles per CU at most, and the smaller tile's
+    // shorter workgroup lifetime is what a pass costs
+    if (w % 256 == 0 && h % 32 == 0 && (size_t)w * (size_t)h > ((size_t)1 << 18)) {
         *px = 256;
         *py = 32;
     } else if (w % 128 == 0 && h % 16 == 0) {

@@ -200,3 +200,42 @@ def test_a_4096_lattice_keeps_its_books(engine, oracle_lib):
     a.close()
     b = gpu_run(m, l, t, 6)
     assert np.array_equal(b.labels(), labels)
+
+
+def test_the_4096_sweep_matches_the_serial_engine_in_distribution(engine, oracle_lib):
+    """The measured size against the f64 serial engine, in distribution: the
+    reference's benchmark model (examples/bench.rs:7-25: contact 16/8, V_t 64,
+    lambda_V 2, T 10, Moore), 100 steps. Per-cell statistics do not depend on
+    the lattice's size, so the serial side runs four 256^2 sheets (seeds
+    1-4, ~2 s) against one 4096^2 device sheet of 262144 cells. Criteria
+    as gpu_matches_cpu.rs:48-92: every cell kept, the same mean volume, the
+    volume spread about target within a quarter; and the mean surface within
+    5%."""
+    from glazier_amd.model import CellType, Model, default_cell_types, tiled_labels
+
+    def bench_model(side, seed):
+        c = CellType(target_volume=64.0, lambda_volume=2.0)
+        m = Model(width=side, height=side, contact=[0.0, 16.0, 16.0, 8.0], types=[CellType(), c],
+                  temperature=10.0, neighbour_order=2, seed=seed)
+        n = side // 8
+        return m, tiled_labels(m, 8, n, n, 1), default_cell_types(n * n + 1)
+
+    sc, surf_c = [], []
+    for seed in (1, 2, 3, 4):
+        cpu = oracle.SerialSimulation(*bench_model(256, seed))
+        cpu.step(100)
+        v = cpu.volumes()[1:]
+        assert v.size == 1024 and v.mean() == 64.0
+        sc.append(spread(v, 64.0))
+        surf_c.append(cpu.surfaces()[1:].mean())
+    gpu = gpu_run(*bench_model(4096, 1), 100)
+    vg = gpu.volumes()[1:]
+    assert vg.size == 262144 and (vg >= 1).all() and vg.mean() == 64.0
+    sg = spread(vg, 64.0)
+    surf_g = gpu.surfaces()[1:].mean()
+    a, b = float(np.mean(sc)), sg
+    print(f"[distribution 4096] spread about target: serial {a:.3f}, device {b:.3f}; "
+          f"mean surface: serial {np.mean(surf_c):.2f}, device {surf_g:.2f}", flush=True)
+    assert abs(a - b) / max(a, b) < 0.25, (a, b)
+    assert abs(np.mean(surf_c) - surf_g) / np.mean(surf_c) < 0.05
+    gpu.close()

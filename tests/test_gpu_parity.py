@@ -70,3 +70,28 @@ def test_the_device_refuses_an_odd_lattice(engine):
     with pytest.raises(GhipError) as err:
         HipSimulation(m, l, t)
     assert "even sides" in str(err.value)
+
+
+@pytest.mark.parametrize("name,chunks", [("ref64", (70, 3, 1)), ("fast_medium_typed", (65,)),
+                                         ("rect64x32", (1, 64, 2))])
+def test_a_resident_lattice_matches_across_launch_and_call_boundaries(engine, oracle_lib, name, chunks):
+    # small plain lattices step inside one workgroup, at most 64 steps per
+    # launch: runs that cross those launches and calls match the emulation
+    from glazier_amd import HipSimulation
+
+    m, labels, types, _, _, _ = case(name)
+    steps = sum(chunks)
+    print(f"[parity] resident {name}: {m.width}x{m.height}, chunks {chunks}", flush=True)
+    gpu = HipSimulation(m, labels, types)
+    gpu.set_timing(True)
+    for c in chunks:
+        gpu.step(c)
+    _, launches = gpu.kernel_time("sweep")
+    assert launches == sum((c + 63) // 64 for c in chunks), "the resident pass did not run"
+    emu = oracle.EmuSimulation(m, labels, types)
+    emu.step(steps)
+    assert np.array_equal(gpu.labels(), emu.labels())
+    assert np.array_equal(gpu.cell_state()[1:], emu.cell_state()[1:])
+    assert gpu.counters() == emu.counters()
+    assert gpu.mcs == steps
+    gpu.close()
