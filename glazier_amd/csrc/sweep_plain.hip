@@ -6,7 +6,8 @@
 // as the CPU emulation); only the work is organised for gfx950:
 //
 //  * a workgroup owns a PX x PY tile (256 x 32 sites with 512 lanes above
-//    512^2 where the lattice divides, else 128 x 16 with 256 lanes). Every
+//    512^2 where the lattice divides, 128 x 8 up to 256^2, else 128 x 16,
+//    both with 256 lanes). Every
 //    global read of the workgroup (the PY + 2 tile rows as 16-byte loads,
 //    the halo columns, the acceptance table and its share of the snapshot
 //    fold) is issued before the first wait; rows land in an LDS tile of pitch PX + 4 words, the
@@ -436,9 +437,12 @@ hipError_t launch_sweep_plain2d_resident(const PlainResidentArgs &r, const Plain
 bool plain2d_supported(uint32_t n_thr) { return n_thr <= (uint32_t)kPThr; }
 
 void plain2d_tile(int w, int h, int *px, int *py) {
-    // up to 512^2 a pass is a few tiles per CU at most, and the smaller tile's
-    // shorter workgroup lifetime is what a pass costs
-    if (w % 256 == 0 && h % 32 == 0 && (size_t)w * (size_t)h > ((size_t)1 << 18)) {
+    // up to 512^2 a pass is a few tiles per CU at most, and the smaller
+    // tile's shorter workgroup lifetime is what a pass costs
+    if (w % 128 == 0 && h % 8 == 0 && (size_t)w * (size_t)h <= ((size_t)1 << 16)) {
+        *px = 128;
+        *py = 8;
+    } else if (w % 256 == 0 && h % 32 == 0 && (size_t)w * (size_t)h > ((size_t)1 << 18)) {
         *px = 256;
         *py = 32;
     } else if (w % 128 == 0 && h % 16 == 0) {
@@ -455,6 +459,8 @@ hipError_t launch_sweep_plain2d(const SweepArgs &a, const PlainParams &p, int bl
     plain2d_tile(a.w, a.h, &px, &py);
     if (px == 256)
         hipLaunchKernelGGL((sweep_plain2d_kernel<256, 32, 512>), dim3(blocks), dim3(512), 0, stream, a, p);
+    else if (px == 128 && py == 8)
+        hipLaunchKernelGGL((sweep_plain2d_kernel<128, 8, 256>), dim3(blocks), dim3(256), 0, stream, a, p);
     else if (px == 128)
         hipLaunchKernelGGL((sweep_plain2d_kernel<128, 16, 256>), dim3(blocks), dim3(256), 0, stream, a, p);
     else
