@@ -307,6 +307,39 @@ def configs_bench():
     return out
 
 
+def blueprints_bench():
+    """The reference's published runs through the binary (README.md:206-210,
+    248-250): the `glazier` command line on infection-512.json (200 steps),
+    volume.json and infection-volume.json (64x64x32; 50 and 60 steps), each
+    a child process from a cold start, as the reference times them: its
+    `seconds` runs from before device setup (main.rs:153-157, 293, 317) and
+    so does this binary's. Dumps are left as each blueprint asks."""
+    import subprocess
+    import tempfile
+
+    root = Path(__file__).resolve().parent
+    binary = root / "glazier_amd" / "glazier"
+    published = {"infection-512.json": (9.72, 1.66), "volume.json": (1.60, 0.092),
+                 "infection-volume.json": (1.09, 0.081)}
+    out = {}
+    for name, (cpu_s, dev_s) in published.items():
+        with tempfile.TemporaryDirectory() as tmp:
+            cmd = [str(binary), "--model", str(root / "tests" / "golden" / "blueprints" / name),
+                   "--out", tmp, "--engine", "hip"]
+            done = subprocess.run(cmd, capture_output=True, text=True, timeout=300)
+            if done.returncode != 0:
+                out[name] = {"error": done.stderr.strip()[-300:]}
+                continue
+            summary = json.loads((Path(tmp) / "summary.json").read_text())
+        out[name] = {"steps": summary["steps"], "seconds": summary["seconds"],
+                     "setup_seconds": summary["setup_seconds"],
+                     "site_attempts_per_s": summary["site_attempts_per_second"],
+                     "n_cells": summary["n_cells"],
+                     "reference_device_seconds": dev_s, "reference_serial_seconds": cpu_s,
+                     "speedup_vs_reference_device": dev_s / summary["seconds"]}
+    return out
+
+
 def counter_per_dispatch(args, counter: str, kernel: str):
     """Run this script under `rocprofv3 --pmc <counter>` in a child process
     (the program directly after `--`) and return the counter's mean value per
@@ -474,6 +507,7 @@ def main() -> int:
             line["roofline"]["traffic_detail"] = traffic
     if not args.no_configs:
         line["configs"] = configs_bench()
+        line["blueprints"] = blueprints_bench()
     if not args.no_ensemble:
         line["ensemble"] = ensemble_bench(args.ensemble_mcs)
     leg = None
