@@ -240,8 +240,27 @@ class Blueprint:
                      secretion=[zeros] + [rates(t.secretion) for t in self.types],
                      uptake=[zeros] + [rates(t.uptake) for t in self.types],
                      chemotaxis=[zeros] + [rates(t.chemotaxis) for t in self.types],
-                     has_nematic=self.nematic_field is not None and any(
-                         t.lambda_nematic != 0.0 for t in self.types))
+                     nematic=self.nematic())
+
+    def nematic(self) -> Optional[np.ndarray]:
+        """The nematic field the description names (blueprint.rs:347-358): a .npy
+        of shape (height, width, 2) or (depth, height, width, 2), f8 or f4,
+        resolved against the description's directory, as (sites, 2) f64."""
+        if self.nematic_field is None:
+            return None
+        path = Path(self.nematic_field)
+        if not path.is_absolute():
+            path = self.base / path
+        raw = np.load(path, allow_pickle=False)
+        if raw.ndim < 1 or raw.shape[-1] != 2:
+            raise ValueError(f"{path}: the last axis is {raw.shape[-1:]}, want 2")
+        if raw.dtype not in (np.float64, np.float32):
+            raise ValueError(f"{path}: a field of dtype {raw.dtype} is not read")
+        pairs = raw.astype(np.float64).reshape(-1, 2)
+        sites = self.width * self.height * self.depth
+        if pairs.shape[0] != sites:
+            raise ValueError(f"{path}: shape {raw.shape} holds {pairs.shape[0]} sites for a lattice of {sites}")
+        return pairs
 
     def simulation(self):
         """(model, labels, cell types) as Blueprint::simulation builds them."""

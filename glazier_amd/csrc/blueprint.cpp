@@ -720,14 +720,25 @@ bool npy_write_f64(const std::string &path, const double *data, uint32_t w, uint
     return npy_write(path, data, 8, "<f8", w, h, d, error);
 }
 
-// npy::read_labels (npy.rs:146-176): <u4, <i4 (non-negative), <i8/<u8.
-bool npy_read_labels(const std::string &path, std::vector<uint32_t> *out, std::string *error) {
-    std::ifstream f(path, std::ios::binary);
-    if (!f) {
+namespace {
+// The parts of a .npy file the readers need (npy.rs:96-144): dtype string,
+// shape, and the data that follows the header.
+struct NpyFile {
+    std::string bytes;
+    std::string descr;
+    std::vector<size_t> shape;
+    size_t count = 1;
+    size_t offset = 0;
+};
+
+bool npy_open(const std::string &path, NpyFile *f, std::string *error) {
+    std::ifstream in(path, std::ios::binary);
+    if (!in) {
         *error = "cannot read " + path;
         return false;
     }
-    std::string bytes((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
+    f->bytes.assign((std::istreambuf_iterator<char>(in)), std::istreambuf_iterator<char>());
+    const std::string &bytes = f->bytes;
     if (bytes.size() < 10 || bytes.compare(0, 6, "\x93NUMPY") != 0) {
         *error = "not a .npy file";
         return false;
@@ -737,9 +748,17 @@ bool npy_read_labels(const std::string &path, std::vector<uint32_t> *out, std::s
         len = (uint8_t)bytes[8] | ((size_t)(uint8_t)bytes[9] << 8);
         start = 10;
     } else {
+        if (bytes.size() < 12) {
+            *error = "not a .npy file";
+            return false;
+        }
         len = (uint8_t)bytes[8] | ((size_t)(uint8_t)bytes[9] << 8) | ((size_t)(uint8_t)bytes[10] << 16) |
               ((size_t)(uint8_t)bytes[11] << 24);
         start = 12;
+    }
+    if (start + len > bytes.size()) {
+        *error = "the .npy header runs past the file";
+        return false;
     }
     std::string header = bytes.substr(start, len);
     if (header.find("'fortran_order': True") != std::string::npos) {
@@ -748,19 +767,35 @@ bool npy_read_labels(const std::string &path, std::vector<uint32_t> *out, std::s
     }
     size_t dp = header.find("'descr':");
     size_t q1 = header.find('\'', dp + 8), q2 = header.find('\'', q1 + 1);
-    std::string descr = header.substr(q1 + 1, q2 - q1 - 1);
     size_t sp = header.find('(', header.find("'shape':")), se = header.find(')', sp);
-    std::string shape = header.substr(sp + 1, se - sp - 1);
-    size_t count = 1;
-    std::stringstream ss(shape);
+    if (dp == std::string::npos || q2 == std::string::npos || sp == std::string::npos ||
+        se == std::string::npos) {
+        *error = "the .npy header has no dtype or shape";
+        return false;
+    }
+    f->descr = header.substr(q1 + 1, q2 - q1 - 1);
+    std::stringstream ss(header.substr(sp + 1, se - sp - 1));
     std::string tok;
+    f->count = 1;
     while (std::getline(ss, tok, ',')) {
         size_t a = tok.find_first_not_of(' ');
         if (a == std::string::npos) continue;
-        count *= (size_t)std::stoull(tok.substr(a));
+        f->shape.push_back((size_t)std::stoull(tok.substr(a)));
+        f->count *= f->shape.back();
     }
-    const char *data = bytes.data() + start + len;
-    size_t avail = bytes.size() - start - len;
+    f->offset = start + len;
+    return true;
+}
+}  // namespace
+
+// npy::read_labels (npy.rs:146-176): <u4, <i4 (non-negative), <i8/<u8.
+bool npy_read_labels(const std::string &path, std::vector<uint32_t> *out, std::string *error) {
+    NpyFile f;
+    if (!npy_open(path, &f, error)) return false;
+    const size_t count = f.count;
+    const char *data = f.bytes.data() + f.offset;
+    const size_t avail = f.bytes.size() - f.offset;
+    const std::string &descr = f.descr;
     out->resize(count);
     if (descr == "<u4" || descr == "<i4") {
         if (avail < count * 4) {
@@ -793,6 +828,40 @@ bool npy_read_labels(const std::string &path, std::vector<uint32_t> *out, std::s
     } else {
         *error = "labels of dtype " + descr + " are not read";
         return false;
+    }
+    return true;
+}
+
+// npy::read_pairs (npy.rs:188-214): a last axis of 2, <f8 or <f4; the values
+// in file order, two per site.
+bool npy_read_pairs(const std::string &path, std::vector<double> *out, std::string *error) {
+    NpyFile f;
+    if (!npy_open(path, &f, error)) return false;
+    if (f.shape.empty() || f.shape.back() != 2) {
+        *error = "the last axis is " + (f.shape.empty() ? std::string("missing") : std::to_string(f.shape.back())) +
+                 ", want 2";
+        return false;
+    }
+    const char *data = f.bytes.data() + f.offset;
+    const size_t avail = f.bytes.size() - f.offset;
+    const size_t elem = f.descr == "<f8" ? 8 : f.descr == "<f4" ? 4 : 0;
+    if (!elem) {
+        *error = "a field of dtype " + f.descr + " is not read";
+        return false;
+    }
+    if (avail < f.count * elem) {
+        *error = "the .npy file is shorter than its shape";
+        return false;
+    }
+    out->resize(f.count);
+    for (size_t i = 0; i < f.count; ++i) {
+        if (elem == 8) {
+            std::memcpy(&(*out)[i], data + 8 * i, 8);
+        } else {
+            float v;
+            std::memcpy(&v, data + 4 * i, 4);
+            (*out)[i] = (double)v;
+        }
     }
     return true;
 }

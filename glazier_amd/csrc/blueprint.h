@@ -103,5 +103,6 @@ bool npy_write_u32(const std::string &path, const uint32_t *data, uint32_t w, ui
 bool npy_write_f64(const std::string &path, const double *data, uint32_t w, uint32_t h, uint32_t d,
                    std::string *error);
 bool npy_read_labels(const std::string &path, std::vector<uint32_t> *out, std::string *error);
+bool npy_read_pairs(const std::string &path, std::vector<double> *out, std::string *error);
 
 }  // namespace ghip

@@ -10,6 +10,7 @@
 // (glazier-core/src/cpu.rs:11-59, 620-725).
 //
 // Per-label state is `cw` int64 words: word 0 packs volume * 2^32 + surface;
+// words 10, 11 (with the nematic coupling) are the field sums in units of 2^-24;
 // words 1..9 (when moments are kept) are the sums of ux, uy, uz, ux^2, uy^2,
 // uz^2, ux*uy, ux*uz, uy*uz over the label's sites, u being the site's
 // minimum-image offset from the label's integer anchor. Integer sums are
@@ -89,7 +90,7 @@ __device__ __forceinline__ void recount_body(const CellArgs &a, size_t n_sites, 
                                                       int moments_only, int gate, const int lb, const int gb) {
     if (gate >= 0 && a.flags[gate] == 0) return;
     __shared__ uint32_t keys[kSlots];
-    __shared__ unsigned long long vals[kSlots * 10];
+    __shared__ unsigned long long vals[kSlots * 12];
     const int cw = a.cw;
     for (int s = threadIdx.x; s < kSlots; s += blockDim.x) keys[s] = 0xFFFFFFFFu;
     for (int s = threadIdx.x; s < kSlots * cw; s += blockDim.x) vals[s] = 0;
@@ -102,9 +103,9 @@ __device__ __forceinline__ void recount_body(const CellArgs &a, size_t n_sites, 
         uint32_t l = label_id(word);
         int x, y, z;
         coords(i, a.w, a.h, &x, &y, &z);
-        int64_t c[10];
+        int64_t c[12];
 #pragma unroll
-        for (int q = 0; q < 10; ++q) c[q] = 0;
+        for (int q = 0; q < 12; ++q) c[q] = 0;
         if (!moments_only) {
             // neighbour offsets are within one site on every axis
             int bonds = 0;
@@ -129,6 +130,10 @@ __device__ __forceinline__ void recount_body(const CellArgs &a, size_t n_sites, 
             c[7] = ux * uy;
             c[8] = ux * uz;
             c[9] = uy * uz;
+            if (cw > 10) {
+                c[10] = a.nematic[2 * i];
+                c[11] = a.nematic[2 * i + 1];
+            }
         }
         uint32_t slot = hash32(l) & (kSlots - 1);
         int found = -1;
@@ -141,7 +146,7 @@ __device__ __forceinline__ void recount_body(const CellArgs &a, size_t n_sites, 
             slot = (slot + 1) & (kSlots - 1);
         }
 #pragma unroll
-        for (int q = 0; q < 10; ++q) {
+        for (int q = 0; q < 12; ++q) {
             if (q >= cw || c[q] == 0) continue;
             if (found >= 0)
                 atomicAdd(&vals[found * cw + q], (unsigned long long)c[q]);

@@ -137,6 +137,12 @@ hipError_t launch_recount(const CellArgs &a, size_t n_sites, hipStream_t s) {
     return hipGetLastError();
 }
 
+hipError_t launch_recount_moments(const CellArgs &a, size_t n_sites, hipStream_t s) {
+    hipLaunchKernelGGL(zero_moments_kernel, dim3(blocks_for(a.capacity, 256)), dim3(256), 0, s, a);
+    recount_gated(a, n_sites, 1, -1, s);
+    return hipGetLastError();
+}
+
 // Anchors at each label's first site, then one recentring onto the rounded
 // centroid. The state's volume words must be current.
 hipError_t launch_initial_anchors(const CellArgs &a, size_t n_sites, hipStream_t s) {

@@ -117,14 +117,28 @@ int main(int argc, char **argv) {
         std::fprintf(stderr, "cannot create %s\n", out_dir.c_str());
         return 1;
     }
-    if (bp.has_nematic_field) {
-        std::fprintf(stderr, "the nematic coupling is not priced by the hip engine\n");
-        return 1;
-    }
-
     // Blueprint::simulation (blueprint.rs:336-375)
     const std::string base = dirname_of(model_path);
     ghip_model m = bp.model();
+    std::vector<double> nematic;
+    if (bp.has_nematic_field) {
+        // blueprint.rs:347-358: resolved against the description's directory
+        const std::string path = bp.nematic_field.size() && bp.nematic_field[0] == '/'
+                                     ? bp.nematic_field
+                                     : join(base, bp.nematic_field);
+        if (!npy_read_pairs(path, &nematic, &err)) {
+            std::fprintf(stderr, "%s: %s\n", path.c_str(), err.c_str());
+            return 1;
+        }
+        const size_t sites = (size_t)m.width * m.height * m.depth;
+        if (nematic.size() != 2 * sites) {
+            std::fprintf(stderr, "%s: holds %zu sites for a lattice of %zu\n", path.c_str(),
+                         nematic.size() / 2, sites);
+            return 1;
+        }
+        for (uint32_t t = 0; t < m.n_types; ++t)
+            if (m.types[t].lambda_nematic != 0.0) m.has_nematic = 1;
+    }
     std::vector<uint32_t> labels;
     size_t cells = 0;
     if (bp.has_labels) {
@@ -155,8 +169,8 @@ int main(int argc, char **argv) {
 
     const auto began = std::chrono::steady_clock::now();
     ghip_sim *sim = nullptr;
-    ghip_status st = ghip_create(&m, labels.data(), labels.size(), types.data(), types.size(), nullptr,
-                                 nullptr, &sim);
+    ghip_status st = ghip_create_nematic(&m, labels.data(), labels.size(), types.data(), types.size(),
+                                         nullptr, nullptr, nematic.empty() ? nullptr : nematic.data(), &sim);
     if (st != GHIP_OK) {
         std::fprintf(stderr, "%s\n", ghip_last_error());
         return 1;

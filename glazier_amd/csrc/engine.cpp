@@ -58,7 +58,7 @@ void build_tables(ghip_sim *s) {
         }
     for (int a = 0; a < nt * nt; ++a)
         if (!is_integer(m.contact[a])) integral = false;
-    bool conn = false, len = false, chem = false, mot = false, drift = false, pop = false;
+    bool conn = false, len = false, chem = false, mot = false, drift = false, pop = false, nem = false;
     for (int i = 0; i < nt; ++i) {
         const ghip_cell_type &c = m.types[i];
         t.lam_v[i] = (float)c.lambda_volume;
@@ -79,6 +79,8 @@ void build_tables(ghip_sim *s) {
         }
         t.lam_len[i] = c.lambda_length;
         t.tgt_len[i] = c.target_length;
+        t.lam_nem[i] = m.has_nematic ? c.lambda_nematic : 0.0;
+        if (m.has_nematic && c.lambda_nematic != 0.0) nem = true;
         t.div_vol[i] = c.division_volume;
         t.death[i] = c.death_rate;
         t.connected[i] = c.connected ? 1u : 0u;
@@ -88,7 +90,7 @@ void build_tables(ghip_sim *s) {
         if (c.external[0] != 0.0 || c.external[1] != 0.0 || c.external[2] != 0.0) drift = true;
         if (c.division_volume > 0.0 || c.death_rate > 0.0) pop = true;
     }
-    if (len || chem || mot || drift) integral = false;
+    if (len || chem || mot || drift || nem) integral = false;
     const double lim = m.temperature * 24.0 * std::log(2.0);
     if (integral && lim + 1.0 < (double)kMaxThr) {
         t.n_thr = (uint32_t)std::floor(lim) + 1;
@@ -154,9 +156,12 @@ void build_tables(ghip_sim *s) {
     if (chem) s->sweep_flags |= kChemotaxis;
     if (mot) s->sweep_flags |= kMotility;
     if (drift) s->sweep_flags |= kDrift;
+    if (nem) s->sweep_flags |= kNematic;
     s->has_pop = pop;
     s->has_mot = mot;
-    s->tracks_moments = len || pop;
+    s->has_nem = nem;
+    // model.rs:127-130: the length term, the nematic term and division read the moments
+    s->tracks_moments = len || pop || nem;
     if (s->tracks_moments) s->sweep_flags |= kMoments;
     s->full = conn || len || chem || mot || drift || s->tracks_moments;
 }
@@ -247,6 +252,21 @@ PlainParams plain_params(const ghip_sim *s) {
 // steps per launch so no launch runs long. Pass and step counters advance as
 // the per-pass driver's would; the record rotation lands where it would.
 constexpr uint64_t kResidentChunk = 64;
+
+// The nematic field as the device holds it: each value rounded to the nearest
+// multiple of 2^-24 (ties away from zero), |value| below 128 so it fits int32.
+bool nematic_to_fixed(const double *q, size_t n_sites, std::vector<int32_t> *out, std::string *err) {
+    out->resize(2 * n_sites);
+    for (size_t i = 0; i < 2 * n_sites; ++i) {
+        if (!(std::fabs(q[i]) < 128.0)) {
+            *err = "nematic value " + std::to_string(q[i]) + " at site " + std::to_string(i / 2) +
+                   " lies outside (-128, 128)";
+            return false;
+        }
+        (*out)[i] = (int32_t)std::llround(q[i] * kNematicScale);
+    }
+    return true;
+}
 
 ghip_status run_resident(ghip_sim *s, uint64_t steps) {
     const PlainParams pp = plain_params(s);
@@ -408,6 +428,12 @@ const char *ghip_last_error(void) { return g_last_error.c_str(); }
 ghip_status ghip_create(const ghip_model *model, const uint32_t *labels, size_t n_sites,
                         const uint8_t *cell_type, size_t n_labels, const double *fields,
                         const double *activity, ghip_sim **out) {
+    return ghip_create_nematic(model, labels, n_sites, cell_type, n_labels, fields, activity, nullptr, out);
+}
+
+ghip_status ghip_create_nematic(const ghip_model *model, const uint32_t *labels, size_t n_sites,
+                                const uint8_t *cell_type, size_t n_labels, const double *fields,
+                                const double *activity, const double *nematic, ghip_sim **out) {
     g_last_error.clear();
     if (!model || !labels || !cell_type || !out) return fail(GHIP_BAD_ARG, "a required pointer is null");
     *out = nullptr;
@@ -421,9 +447,9 @@ ghip_status ghip_create(const ghip_model *model, const uint32_t *labels, size_t 
                       m.width, m.height, m.depth);
         return fail(GHIP_ODD_LATTICE, buf);
     }
-    if (m.has_nematic)
-        return fail(GHIP_UNSUPPORTED, "the nematic coupling is not priced by the hip engine; "
-                                      "refusing rather than dropping it");
+    if (m.has_nematic && !nematic)
+        return fail(GHIP_BAD_ARG, "the model couples to a nematic field that was not given "
+                                  "(ghip_create_nematic takes it)");
     if (m.n_types < 1 || m.n_types > GHIP_MAX_TYPES)
         return fail(GHIP_UNSUPPORTED, "the hip engine takes 1 to 16 cell types");
     if (m.n_species > GHIP_MAX_SPECIES) return fail(GHIP_UNSUPPORTED, "the hip engine takes up to 4 species");
@@ -484,7 +510,15 @@ ghip_status ghip_create(const ghip_model *model, const uint32_t *labels, size_t 
     }
     build_tables(s);
     choose_tiles(s);
-    s->cw = s->tracks_moments ? 10 : 1;
+    s->cw = s->tracks_moments ? (s->has_nem ? 12 : 10) : 1;
+    std::vector<int32_t> nq;
+    if (s->has_nem) {
+        std::string err;
+        if (!nematic_to_fixed(nematic, N, &nq, &err)) {
+            delete s;
+            return fail(GHIP_BAD_ARG, err);
+        }
+    }
     uint64_t cap = n_labels;
     if (s->has_pop) cap = std::max<uint64_t>(2 * n_labels + 1024, n_labels + N / 16);
     cap = std::min<uint64_t>(cap, kIdMask - 1);
@@ -531,6 +565,10 @@ ghip_status ghip_create(const ghip_model *model, const uint32_t *labels, size_t 
     CREATE_TRY(hipMalloc(&s->plan, (size_t)cap));
     CREATE_TRY(hipMalloc(&s->geo, (size_t)cap * 6 * 8));
     if (s->sweep_flags & kLength) CREATE_TRY(hipMalloc(&s->len_before, (size_t)cap * 8));
+    if (s->has_nem) {
+        CREATE_TRY(hipMalloc(&s->nematic, N * 2 * 4));
+        CREATE_TRY(hipMemcpy(s->nematic, nq.data(), N * 2 * 4, hipMemcpyHostToDevice));
+    }
     CREATE_TRY(hipMalloc(&s->flags, 16));
     CREATE_TRY(hipMalloc(&s->type_copy, (size_t)cap * 4));
     CREATE_TRY(hipMalloc(&s->anchor_copy, (size_t)cap * 3 * 4));
@@ -727,6 +765,26 @@ ghip_status ghip_field(const ghip_sim *s, uint32_t species, double *out, size_t 
     return GHIP_OK;
 }
 
+ghip_status ghip_set_nematic_field(ghip_sim *s, const double *q, size_t n) {
+    if (!s || !q) return fail(GHIP_BAD_ARG, "null argument");
+    if (n != s->N)
+        return fail(GHIP_BAD_ARG, "nematic field has " + std::to_string(n) + " sites, the lattice " +
+                                      std::to_string(s->N));
+    if (!s->has_nem) return fail(GHIP_BAD_ARG, "the model has no nematic coupling to update");
+    std::vector<int32_t> nq;
+    std::string err;
+    if (!nematic_to_fixed(q, n, &nq, &err)) return fail(GHIP_BAD_ARG, err);
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    HIP_TRY(hipMemcpy(s->nematic, nq.data(), n * 2 * 4, hipMemcpyHostToDevice));
+    // every label's field sum is recounted: the record is brought current
+    // (A += the last pass's delta), then its moment words are resummed
+    const uint64_t p = s->pass;
+    HIP_TRY(launch_consolidate(s->A[p % 2], s->Dl[(p + 2) % 3], s->n_labels_dev, s->cw, s->cap, s->stream));
+    HIP_TRY(launch_recount_moments(s->cell_args(), s->N, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return GHIP_OK;
+}
+
 ghip_status ghip_set_field(ghip_sim *s, uint32_t species, const double *values, size_t n) {
     if (!s || !values) return fail(GHIP_BAD_ARG, "null argument");
     if (species >= s->m.n_species) return fail(GHIP_BAD_ARG, "no such species");
@@ -760,7 +818,8 @@ ghip_status ghip_counters(const ghip_sim *s, uint64_t *attempted, uint64_t *acce
 
 ghip_status ghip_cell_state(const ghip_sim *s, int64_t *out, size_t n, uint32_t *words_per_label) {
     if (!s || !words_per_label) return fail(GHIP_BAD_ARG, "null argument");
-    const uint32_t words = s->tracks_moments ? 11 : 2;
+    // volume and surface, then every further record word (moments, nematic sums)
+    const uint32_t words = (uint32_t)s->cw + 1;
     *words_per_label = words;
     if (!out) return GHIP_OK;
     std::vector<int64_t> cur;

@@ -106,6 +106,23 @@ GHD double cov_largest(const int64_t *m, int64_t n, bool three_d, double *axis) 
     return best > 0.0 ? best : 0.0;
 }
 
+// In-plane anisotropy (Moments::anisotropy, glazier-core/src/moments.rs:131-141)
+// of the scaled covariance, which it does not depend on: (C_xx - C_yy)/tr and
+// 2 C_xy/tr with tr = C_xx + C_yy; zero below two sites or at tr = 0 (an
+// integer trace is 0 or at least 1, i.e. at least 1/n^2 unscaled).
+GHD void anisotropy(const int64_t *m, int64_t n, double *a0, double *a1) {
+    *a0 = 0.0;
+    *a1 = 0.0;
+    if (n < 2) return;
+    const int64_t cxx = n * m[3] - m[0] * m[0];
+    const int64_t cyy = n * m[4] - m[1] * m[1];
+    const int64_t cxy = n * m[6] - m[0] * m[1];
+    const int64_t tr = cxx + cyy;
+    if (tr <= 0) return;
+    *a0 = (double)(cxx - cyy) / (double)tr;
+    *a1 = 2.0 * (double)cxy / (double)tr;
+}
+
 // Major axis 4 sqrt(lambda_max(cov)) of n sites with sums m.
 GHD double length_of(const int64_t *m, int64_t n, bool three_d) {
     if (n < 2) return 0.0;

@@ -27,6 +27,7 @@ struct DeviceTables {
     float up[kMaxTypes][kMaxSpecies];
     double lam_len[kMaxTypes];
     double tgt_len[kMaxTypes];
+    double lam_nem[kMaxTypes];
     double div_vol[kMaxTypes];
     double death[kMaxTypes];
     uint32_t connected[kMaxTypes];
@@ -46,8 +47,14 @@ enum SweepFlags : uint32_t {
     kChemotaxis = 1u << 3,
     kMotility = 1u << 4,
     kDrift = 1u << 5,
-    kMoments = 1u << 6,  // per-label moment sums are kept (length or division)
+    kMoments = 1u << 6,  // per-label moment sums are kept (length, nematic or division)
+    kNematic = 1u << 7,  // the nematic coupling: record words 10, 11 hold the field sums
 };
+
+// The nematic field is held per site as two int32 in units of 2^-24, so a
+// label's field sum is an exact int64 in any order of accumulation.
+constexpr double kNematicScale = 16777216.0;       // 2^24
+constexpr double kNematicUnit = 5.9604644775390625e-8;  // 2^-24
 
 // One colour pass over one lattice.
 struct SweepArgs {
@@ -63,6 +70,7 @@ struct SweepArgs {
     int64_t *g_zero;             // delta buffer of the pass after next, cleared
     const int32_t *anchor;       // 3 per label
     const double *len_before;    // per label: length minus target in this pass's snapshot
+    const int32_t *nematic;      // (Q_xx, Q_xy) per site in units of 2^-24, or null
     const DeviceTables *tables;
     const uint32_t *n_labels;    // device label count
     unsigned long long *accepted; // kAcceptSlots counters
@@ -135,6 +143,7 @@ struct CellArgs {
     uint32_t *daughter;    // parent -> daughter id, capacity entries
     uint8_t *plan;         // capacity entries
     double *geo;           // 6 per label
+    const int32_t *nematic; // (Q_xx, Q_xy) per site in units of 2^-24, or null
     uint32_t *flags;       // [0] event, [1] recentre, [2] overflow, [3] compacted count
     const DeviceTables *tables;
     int w, h, d;
@@ -171,6 +180,9 @@ hipError_t launch_fill_u32(uint32_t *p, size_t n, uint32_t v, hipStream_t s);
 hipError_t launch_consolidate(int64_t *a, int64_t *e, const uint32_t *n_labels, int cw, size_t cap,
                               hipStream_t s);
 hipError_t launch_recount(const CellArgs &a, size_t n_sites, hipStream_t s);
+// Recount of the moment words (1..cw-1, nematic sums included) about the
+// current anchors, volume and surface kept.
+hipError_t launch_recount_moments(const CellArgs &a, size_t n_sites, hipStream_t s);
 hipError_t launch_initial_anchors(const CellArgs &a, size_t n_sites, hipStream_t s);
 hipError_t launch_recentre_check(const CellArgs &a, size_t n_sites, hipStream_t s);
 hipError_t launch_population(const CellArgs &a, size_t n_sites, uint32_t *type_copy,

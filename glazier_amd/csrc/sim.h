@@ -66,6 +66,8 @@ struct ghip_sim {
     uint8_t *plan = nullptr;
     double *geo = nullptr;
     double *len_before = nullptr;  // per label, when the length term is on
+    int32_t *nematic = nullptr;    // (Q_xx, Q_xy) per site in units of 2^-24, with the coupling
+    bool has_nem = false;
     uint32_t *flags = nullptr;
     uint32_t *type_copy = nullptr;
     int32_t *anchor_copy = nullptr;
@@ -107,7 +109,8 @@ struct ghip_sim {
         if (stream) (void)hipStreamSynchronize(stream);
         void *bufs[] = {labels[0], labels[1], act[0], act[1], field_base[0], field_base[1], A[0],
                         A[1], Dl[0], Dl[1], Dl[2], anchor, ltype, scratch, daughter, plan, geo,
-                        flags, type_copy, anchor_copy, n_labels_dev, tables, accepted, traj, len_before};
+                        flags, type_copy, anchor_copy, n_labels_dev, tables, accepted, traj, len_before,
+                        nematic};
         for (void *p : bufs)
             if (p) (void)hipFree(p);
         for (auto *list : {&pending, &spare})
@@ -129,6 +132,7 @@ struct ghip_sim {
         c.daughter = daughter;
         c.plan = plan;
         c.geo = geo;
+        c.nematic = nematic;
         c.flags = flags;
         c.tables = tables;
         c.w = W;
@@ -151,6 +155,7 @@ struct ghip_sim {
         a.act_prev = a.act;
         a.anchor = anchor;
         a.len_before = len_before;
+        a.nematic = nematic;
         a.tables = tables;
         a.n_labels = n_labels_dev;
         a.accepted = accepted;

@@ -303,6 +303,42 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &a, const int blk) {
                     dh = dh + (float)(tab->lam_len[ty] * (now * now - before * before));
                 }
             }
+            if (FULL && (a.flags & kNematic)) {
+                // -lambda (F' . a' - F . a) (glazier-core/src/cpu.rs:315-332): F the
+                // label's field sum (words 10, 11), a its anisotropy, before and
+                // after the site moves
+                const int64_t q0 = a.nematic[2 * site], q1 = a.nematic[2 * site + 1];
+                for (int side = 0; side < 2; ++side) {
+                    const uint32_t id = side == 0 ? oid : nid;
+                    const uint32_t ty = side == 0 ? eto : etn;
+                    if (id == 0 || tab->lam_nem[ty] == 0.0) continue;
+                    const int sign = side == 0 ? -1 : 1;
+                    const size_t base = (size_t)id * cw;
+                    int64_t m[9], after[9];
+                    for (int q = 0; q < 9; ++q) m[q] = a.a_cur[base + 1 + q] + a.e_prev[base + 1 + q];
+                    const int64_t f0 = a.a_cur[base + 10] + a.e_prev[base + 10];
+                    const int64_t f1 = a.a_cur[base + 11] + a.e_prev[base + 11];
+                    const int64_t n = unpack_vol(side == 0 ? pv_old : pv_new);
+                    const int64_t ux = fold_int(gx - a.anchor[3 * id], a.w);
+                    const int64_t uy = fold_int(gy - a.anchor[3 * id + 1], a.h);
+                    const int64_t uz = fold_int(gz - a.anchor[3 * id + 2], a.d);
+                    after[0] = m[0] + sign * ux;
+                    after[1] = m[1] + sign * uy;
+                    after[2] = m[2] + sign * uz;
+                    after[3] = m[3] + sign * ux * ux;
+                    after[4] = m[4] + sign * uy * uy;
+                    after[5] = m[5] + sign * uz * uz;
+                    after[6] = m[6] + sign * ux * uy;
+                    after[7] = m[7] + sign * ux * uz;
+                    after[8] = m[8] + sign * uy * uz;
+                    double a0, a1, b0, b1;
+                    anisotropy(m, n, &a0, &a1);
+                    anisotropy(after, n + sign, &b0, &b1);
+                    const double before = (double)f0 * a0 + (double)f1 * a1;
+                    const double now = (double)(f0 + sign * q0) * b0 + (double)(f1 + sign * q1) * b1;
+                    dh = dh + (float)(-tab->lam_nem[ty] * ((now - before) * kNematicUnit));
+                }
+            }
             if (FULL && (a.flags & kChemotaxis) && nid != 0) {
                 const size_t ssite = ((size_t)sz * a.h + sy) * (size_t)a.w + sx;
                 for (int s = 0; s < a.n_species; ++s) {
@@ -380,6 +416,12 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &a, const int blk) {
                             reinterpret_cast<unsigned long long *>(a.f_acc + (size_t)id * cw + 1);
                         for (int q = 0; q < 9; ++q)
                             if (dm[q] != 0) atomicAdd(dst + q, (unsigned long long)dm[q]);
+                        if (a.flags & kNematic) {
+                            const int64_t d0 = sg * (int64_t)a.nematic[2 * site];
+                            const int64_t d1 = sg * (int64_t)a.nematic[2 * site + 1];
+                            if (d0 != 0) atomicAdd(dst + 9, (unsigned long long)d0);
+                            if (d1 != 0) atomicAdd(dst + 10, (unsigned long long)d1);
+                        }
                     }
                 }
                 if (FULL && (a.flags & kMotility)) {

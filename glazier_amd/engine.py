@@ -60,6 +60,10 @@ def library() -> ctypes.CDLL:
     lib.ghip_create.argtypes = [P(CModel), P(ctypes.c_uint32), ctypes.c_size_t, P(ctypes.c_uint8),
                                 ctypes.c_size_t, P(ctypes.c_double), P(ctypes.c_double),
                                 P(ctypes.c_void_p)]
+    lib.ghip_create_nematic.argtypes = [P(CModel), P(ctypes.c_uint32), ctypes.c_size_t,
+                                        P(ctypes.c_uint8), ctypes.c_size_t, P(ctypes.c_double),
+                                        P(ctypes.c_double), P(ctypes.c_double), P(ctypes.c_void_p)]
+    lib.ghip_set_nematic_field.argtypes = [sim, P(ctypes.c_double), ctypes.c_size_t]
     lib.ghip_destroy.argtypes = [sim]
     lib.ghip_step.argtypes = [sim, ctypes.c_uint64]
     lib.ghip_mcs.argtypes = [sim]
@@ -123,9 +127,21 @@ class HipSimulation:
             a = np.ascontiguousarray(activity, dtype=np.float64).reshape(-1)
             aptr = _ptr(a, ctypes.c_double)
         handle = ctypes.c_void_p()
-        _check(lib.ghip_create(ctypes.byref(self._c), _ptr(lab, ctypes.c_uint32), lab.size,
-                               _ptr(ct, ctypes.c_uint8), ct.size, fptr, aptr, ctypes.byref(handle)))
+        q = model.nematic_pairs() if model.has_nematic else None
+        if q is not None:
+            _check(lib.ghip_create_nematic(ctypes.byref(self._c), _ptr(lab, ctypes.c_uint32), lab.size,
+                                           _ptr(ct, ctypes.c_uint8), ct.size, fptr, aptr,
+                                           _ptr(q, ctypes.c_double), ctypes.byref(handle)))
+        else:
+            _check(lib.ghip_create(ctypes.byref(self._c), _ptr(lab, ctypes.c_uint32), lab.size,
+                                   _ptr(ct, ctypes.c_uint8), ct.size, fptr, aptr, ctypes.byref(handle)))
         self._h = handle
+
+    def set_nematic_field(self, q: np.ndarray) -> None:
+        """GpuSimulation::set_nematic_field (glazier-cuda/src/lib.rs:1199-1221):
+        one (Q_xx, Q_xy) pair per site."""
+        v = np.ascontiguousarray(q, dtype=np.float64).reshape(-1)
+        _check(library().ghip_set_nematic_field(self._h, _ptr(v, ctypes.c_double), v.size // 2))
 
     def close(self) -> None:
         if getattr(self, "_h", None) and self._h.value:
@@ -204,7 +220,8 @@ class HipSimulation:
 
     def cell_state(self) -> np.ndarray:
         """Per label: volume, surface, then the nine integer moment sums when
-        the engine keeps them; shape (n_labels, words)."""
+        the engine keeps them and the two nematic field sums (units of 2^-24)
+        with the coupling; shape (n_labels, words)."""
         words = ctypes.c_uint32()
         lib = library()
         _check(lib.ghip_cell_state(self._h, None, 0, ctypes.byref(words)))

@@ -11,7 +11,7 @@ from __future__ import annotations
 
 import ctypes
 from dataclasses import dataclass, field
-from typing import List, Sequence
+from typing import List, Optional, Sequence
 
 import numpy as np
 
@@ -117,7 +117,24 @@ class Model:
     uptake: List[List[float]] = field(default_factory=list)
     chemotaxis: List[List[float]] = field(default_factory=list)
     replica: int = 0
-    has_nematic: bool = False
+    # Model::nematic (model.rs:70-75): (Q_xx, Q_xy) per site, shape (sites, 2)
+    nematic: Optional[np.ndarray] = None
+
+    @property
+    def has_nematic(self) -> bool:
+        """Model::has_nematic (model.rs:120-124): a field is present and some
+        type couples to it."""
+        return self.nematic is not None and len(self.nematic) > 0 and any(
+            t.lambda_nematic != 0.0 for t in self.types)
+
+    def nematic_pairs(self) -> Optional[np.ndarray]:
+        """The nematic field as a contiguous f64 array of (sites, 2), or None."""
+        if self.nematic is None:
+            return None
+        q = np.ascontiguousarray(self.nematic, dtype=np.float64).reshape(-1, 2)
+        if q.shape[0] != self.sites:
+            raise ValueError(f"the nematic field has {q.shape[0]} sites for a lattice of {self.sites}")
+        return q
 
     @property
     def n_types(self) -> int:
@@ -134,7 +151,7 @@ class Model:
         return (self.height, self.width)
 
     def validate(self) -> None:
-        """Model::validate (glazier-core/src/model.rs:160-232), nematic aside."""
+        """Model::validate (glazier-core/src/model.rs:160-232)."""
         n = self.n_types
         if n == 0:
             raise ValueError("a model needs at least the medium type")
@@ -159,6 +176,8 @@ class Model:
             for index, row in enumerate(table):
                 if row and len(row) != s:
                     raise ValueError(f"type {index} states {len(row)} {name} rates for {s} species")
+        if self.nematic is not None and len(self.nematic) > 0:
+            self.nematic_pairs()  # model.rs:193-197
 
     def to_c(self) -> CModel:
         if self.n_types > MAX_TYPES:

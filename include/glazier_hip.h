@@ -98,11 +98,28 @@ typedef struct ghip_sim ghip_sim;
  * cell_type[0] = 1, glazier-core/src/cpu.rs:203, and the medium then prices
  * contact as that type); `fields` is n_species*n_sites values, species-major,
  * or NULL for every species' initial value; `activity` is n_sites values or
- * NULL for none. Fails with GHIP_ODD_LATTICE on an odd side, and with
- * GHIP_UNSUPPORTED on a nematic model or a term outside this engine. */
+ * NULL for none. Fails with GHIP_ODD_LATTICE on an odd side, with
+ * GHIP_UNSUPPORTED on a term outside this engine, and with GHIP_BAD_ARG on a
+ * model with has_nematic set (its field comes through ghip_create_nematic). */
 ghip_status ghip_create(const ghip_model *model, const uint32_t *labels, size_t n_sites,
                         const uint8_t *cell_type, size_t n_labels, const double *fields,
                         const double *activity, ghip_sim **out);
+
+/* ghip_create for a model with the nematic coupling: from_cpu uploads
+ * Model::nematic with the rest (glazier-cuda/src/lib.rs:886-896, 1016-1020).
+ * `nematic` holds 2 * n_sites values, (Q_xx, Q_xy) per site in lattice order
+ * (glazier-core/src/model.rs:70-75); each must lie in (-128, 128). The
+ * engine holds them in units of 2^-24 (nearest, ties away from zero), so each
+ * cell's field sum is exact in any order; the term prices -lambda F . a
+ * (glazier-core/src/cpu.rs:315-332). Ignored when has_nematic is 0. */
+ghip_status ghip_create_nematic(const ghip_model *model, const uint32_t *labels, size_t n_sites,
+                                const uint8_t *cell_type, size_t n_labels, const double *fields,
+                                const double *activity, const double *nematic, ghip_sim **out);
+
+/* Replaces GpuSimulation::set_nematic_field (glazier-cuda/src/lib.rs:1199-1221):
+ * `q` holds 2 * n values, n = n_sites. Fails when the model has no nematic
+ * coupling. Every cell's field sum is recounted before the next step. */
+ghip_status ghip_set_nematic_field(ghip_sim *sim, const double *q, size_t n);
 
 /* Frees the handle and its device memory (the Drop of GpuSimulation). */
 ghip_status ghip_destroy(ghip_sim *sim);
@@ -149,7 +166,9 @@ ghip_status ghip_counters(const ghip_sim *sim, uint64_t *attempted, uint64_t *ac
 /* Per-label bookkeeping as the engine keeps it, for bit-level comparison:
  * `words` int64 per label in the order volume, surface, then the integer
  * moment sums n-free (sx, sy, sz, sxx, syy, szz, sxy, sxz, syz) about each
- * label's anchor when the length term is on. Returns the word count per
+ * label's anchor when the moments are kept (length, nematic or division),
+ * then with the nematic coupling the field sums (Q_xx, Q_xy) in units of
+ * 2^-24. Returns the word count per
  * label in *words_per_label; `out` may be NULL to ask for it. */
 ghip_status ghip_cell_state(const ghip_sim *sim, int64_t *out, size_t n,
                             uint32_t *words_per_label);

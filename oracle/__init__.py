@@ -86,6 +86,12 @@ def library() -> ctypes.CDLL:
         "serial_substeps": ([u32, P(CModel)], ctypes.c_int),
         "serial_clone": ([v], v),
         "serial_apply_copy": ([v, sz, u32], None),
+        "serial_set_nematic": ([v, P(f64)], None),
+        "serial_nematic_sums": ([v, P(f64)], None),
+        "serial_recounted_nematic": ([v, P(f64)], None),
+        "serial_recounted_anisotropy": ([v, P(f64)], None),
+        "emu_set_nematic": ([v, P(f64)], None),
+        "emu_nematic_fixed": ([f64], ctypes.c_int32),
         "oracle_moments_length": ([P(f64), sz], f64),
         "oracle_principal_axis": ([P(f64), P(f64), P(f64)], None),
         "oracle_hash32": ([u32], u32),
@@ -215,6 +221,9 @@ class SerialSimulation(_Base):
             fp = _p(self._fields, ctypes.c_double)
         self._h = library().serial_create(ctypes.byref(self._c), _p(lab, ctypes.c_uint32),
                                           _p(ct, ctypes.c_uint8), ct.size, fp)
+        q = model.nematic_pairs()
+        if q is not None:
+            library().serial_set_nematic(self._h, _p(q, ctypes.c_double))
 
     def clone(self) -> "SerialSimulation":
         c = SerialSimulation.__new__(SerialSimulation)
@@ -295,6 +304,29 @@ class SerialSimulation(_Base):
     def locally_connected(self, target: int, label: int) -> bool:
         return bool(library().serial_locally_connected(self._h, target, label))
 
+    def set_nematic_field(self, q: np.ndarray) -> None:
+        """Model::nematic replaced (2 values per site); the sums are recounted."""
+        v = np.ascontiguousarray(q, dtype=np.float64).reshape(-1)
+        if v.size != 2 * self.sites:
+            raise ValueError(f"the nematic field has {v.size // 2} sites, the lattice {self.sites}")
+        library().serial_set_nematic(self._h, _p(v, ctypes.c_double))
+
+    def nematic_sums(self) -> np.ndarray:
+        out = np.empty((self.n_labels(), 2), dtype=np.float64)
+        library().serial_nematic_sums(self._h, _p(out, ctypes.c_double))
+        return out
+
+    def recounted_nematic(self) -> np.ndarray:
+        out = np.empty((self.n_labels(), 2), dtype=np.float64)
+        library().serial_recounted_nematic(self._h, _p(out, ctypes.c_double))
+        return out
+
+    def recounted_anisotropy(self) -> np.ndarray:
+        """Each label's (a0, a1) (moments.rs:131-141) from recounted moments."""
+        out = np.empty((self.n_labels(), 2), dtype=np.float64)
+        library().serial_recounted_anisotropy(self._h, _p(out, ctypes.c_double))
+        return out
+
 
 class EmuSimulation(_Base):
     """The CPU emulation of the device schedule (oracle/emu.c)."""
@@ -317,6 +349,15 @@ class EmuSimulation(_Base):
             ap = _p(a, ctypes.c_double)
         self._h = library().emu_create(ctypes.byref(self._c), _p(lab, ctypes.c_uint32),
                                        _p(ct, ctypes.c_uint8), ct.size, fp, ap)
+        q = model.nematic_pairs()
+        if q is not None:
+            library().emu_set_nematic(self._h, _p(q, ctypes.c_double))
+
+    def set_nematic_field(self, q: np.ndarray) -> None:
+        v = np.ascontiguousarray(q, dtype=np.float64).reshape(-1)
+        if v.size != 2 * self.sites:
+            raise ValueError(f"the nematic field has {v.size // 2} sites, the lattice {self.sites}")
+        library().emu_set_nematic(self._h, _p(v, ctypes.c_double))
 
     def cell_state(self) -> np.ndarray:
         lib = library()

@@ -158,6 +158,8 @@ struct emu_sim {
     int64_t *vol, *surf, *mom; /* mom: 9 words per label */
     int64_t *vsnap, *ssnap, *msnap;
     int32_t *anchor;           /* 3 per label */
+    int32_t *nq;               /* nematic field per site, 2 fixed-point values */
+    int64_t *nem, *nsnap;      /* nematic sums per label, 2 fixed-point words */
     float *field[GHIP_MAX_SPECIES];
     float *fscratch;
     float *act, *prev_act;
@@ -176,7 +178,7 @@ struct emu_sim {
     int n_thr;
     uint32_t thr[EMU_MAX_THR];
     float accept_c;
-    int has_conn, has_len, has_chem, has_mot, has_ext, has_pop, tracks_moments;
+    int has_conn, has_len, has_chem, has_mot, has_ext, has_pop, has_nem, tracks_moments;
     uint64_t mcs, attempted, accepted;
 };
 
@@ -224,6 +226,8 @@ static void ensure_cap(emu_sim *e, size_t want) {
     e->ssnap = realloc(e->ssnap, cap * 8);
     e->msnap = realloc(e->msnap, cap * 9 * 8);
     e->anchor = realloc(e->anchor, cap * 3 * 4);
+    e->nem = realloc(e->nem, cap * 2 * 8);
+    e->nsnap = realloc(e->nsnap, cap * 2 * 8);
     e->cap = cap;
 }
 
@@ -246,9 +250,19 @@ static void recount_volume_surface(emu_sim *e) {
     }
 }
 
-/* Integer moment sums of every label about its anchor. */
+/* Integer moment sums of every label about its anchor, and with the
+ * nematic coupling the fixed-point field summed over each label's sites
+ * (kept with the moments, and recounted with them). */
 static void sum_moments(emu_sim *e) {
     memset(e->mom, 0, e->n_labels * 9 * 8);
+    memset(e->nem, 0, e->n_labels * 2 * 8);
+    if (e->has_nem)
+        for (size_t i = 0; i < e->n; ++i) {
+            uint32_t l = e->labels[i];
+            if (l == 0) continue;
+            e->nem[2 * l] += e->nq[2 * i];
+            e->nem[2 * l + 1] += e->nq[2 * i + 1];
+        }
     for (size_t i = 0; i < e->n; ++i) {
         uint32_t l = e->labels[i];
         if (l == 0) continue;
@@ -400,6 +414,23 @@ static double cov_largest(const int64_t *m, int64_t n, int three_d, double axis[
 }
 
 /* Major axis length 4 sqrt(lambda_max(cov)) from integer sums of n sites. */
+/* In-plane anisotropy of the scaled covariance (the reference's
+ * Moments::anisotropy, moments.rs:131-141, is scale-free): (C_xx - C_yy)/tr
+ * and 2 C_xy/tr with tr = C_xx + C_yy, zero below two sites or at tr = 0
+ * (an integer trace is 0 or at least 1, i.e. at least 1/n^2 unscaled). */
+static void aniso(const int64_t *m, int64_t n, double *a0, double *a1) {
+    *a0 = 0.0;
+    *a1 = 0.0;
+    if (n < 2) return;
+    int64_t cxx = n * m[3] - m[0] * m[0];
+    int64_t cyy = n * m[4] - m[1] * m[1];
+    int64_t cxy = n * m[6] - m[0] * m[1];
+    int64_t tr = cxx + cyy;
+    if (tr <= 0) return;
+    *a0 = (double)(cxx - cyy) / (double)tr;
+    *a1 = 2.0 * (double)cxy / (double)tr;
+}
+
 static double length_of(const int64_t *m, int64_t n, int three_d) {
     if (n < 2) return 0.0;
     double lam = cov_largest(m, n, three_d, NULL);
@@ -536,6 +567,7 @@ emu_sim *emu_create(const ghip_model *m, const uint32_t *labels, const uint8_t *
         anchor_from_first_sites(e);
     else
         memset(e->mom, 0, e->n_labels * 9 * 8);
+    memset(e->nem, 0, e->n_labels * 2 * 8);
     return e;
 }
 
@@ -551,6 +583,9 @@ void emu_destroy(emu_sim *e) {
     free(e->ssnap);
     free(e->msnap);
     free(e->anchor);
+    free(e->nq);
+    free(e->nem);
+    free(e->nsnap);
     for (int k = 0; k < GHIP_MAX_SPECIES; ++k) free(e->field[k]);
     free(e->fscratch);
     free(e->act);
@@ -637,6 +672,31 @@ static double length_delta(const emu_sim *e, uint32_t label, int64_t x, int64_t 
     return ct->lambda_length * (now * now - before * before);
 }
 
+/* Change in the nematic energy -lambda F . a (cpu.rs:315-332) of `label`
+ * when the site `site` at (x, y, z) joins (sign +1) or leaves (sign -1): F
+ * the fixed-point field sum (scale 2^24), a the anisotropy, both from the
+ * pass-start snapshot and after the move. */
+static double nematic_delta(const emu_sim *e, uint32_t label, int64_t x, int64_t y, int64_t z,
+                            size_t site, int sign) {
+    int t = e->ctype[label];
+    int64_t n = e->vsnap[label];
+    const int64_t *m = e->msnap + 9 * label;
+    int64_t ux = fold_int(x - e->anchor[3 * label], e->w);
+    int64_t uy = fold_int(y - e->anchor[3 * label + 1], e->h);
+    int64_t uz = fold_int(z - e->anchor[3 * label + 2], e->d);
+    int64_t after[9] = {m[0] + sign * ux,      m[1] + sign * uy,      m[2] + sign * uz,
+                        m[3] + sign * ux * ux, m[4] + sign * uy * uy, m[5] + sign * uz * uz,
+                        m[6] + sign * ux * uy, m[7] + sign * ux * uz, m[8] + sign * uy * uz};
+    double a0, a1, b0, b1;
+    aniso(m, n, &a0, &a1);
+    aniso(after, n + sign, &b0, &b1);
+    const int64_t *f = e->nsnap + 2 * label;
+    int64_t q0 = e->nq[2 * site], q1 = e->nq[2 * site + 1];
+    double before = (double)f[0] * a0 + (double)f[1] * a1;
+    double now = (double)(f[0] + sign * q0) * b0 + (double)(f[1] + sign * q1) * b1;
+    return -e->m.types[t].lambda_nematic * ((now - before) * 5.9604644775390625e-8);
+}
+
 typedef struct {
     float dh;
     int ds_old, ds_new;
@@ -645,7 +705,8 @@ typedef struct {
 /* The energy change of copying `src`'s label into target `t`, from the
  * pass-start state. Terms are added in this order: contact (gained minus
  * lost), volume old, volume new, surface old, surface new, length old,
- * length new, chemotaxis per species, motility, drift. */
+ * length new, nematic old, nematic new, chemotaxis per species, motility,
+ * drift. */
 static price_t price(const emu_sim *e, size_t t, size_t src, int pick) {
     price_t r;
     uint32_t old = e->labels[t], nw = e->labels[src];
@@ -695,6 +756,12 @@ static price_t price(const emu_sim *e, size_t t, size_t src, int pick) {
         if (nw != 0 && e->m.types[tw].lambda_length != 0.0)
             dh = dh + (float)length_delta(e, nw, x, y, z, 1);
     }
+    if (e->has_nem) {
+        if (old != 0 && e->m.types[to].lambda_nematic != 0.0)
+            dh = dh + (float)nematic_delta(e, old, x, y, z, t, -1);
+        if (nw != 0 && e->m.types[tw].lambda_nematic != 0.0)
+            dh = dh + (float)nematic_delta(e, nw, x, y, z, t, 1);
+    }
     if (e->has_chem && nw != 0) {
         for (uint32_t k = 0; k < e->m.n_species; ++k) {
             float chi = e->chi[tw][k];
@@ -722,6 +789,7 @@ double emu_delta_energy(const emu_sim *e, size_t target, size_t source) {
     memcpy(me->vsnap, e->vol, e->n_labels * 8);
     memcpy(me->ssnap, e->surf, e->n_labels * 8);
     memcpy(me->msnap, e->mom, e->n_labels * 9 * 8);
+    memcpy(me->nsnap, e->nem, e->n_labels * 2 * 8);
     memcpy(me->prev_labels, e->labels, e->n * 4);
     memcpy(me->prev_act, e->act, e->n * 4);
     int pick = 0;
@@ -750,6 +818,7 @@ static void one_pass(emu_sim *e, int colour) {
     memcpy(e->vsnap, e->vol, e->n_labels * 8);
     memcpy(e->ssnap, e->surf, e->n_labels * 8);
     if (e->tracks_moments) memcpy(e->msnap, e->mom, e->n_labels * 9 * 8);
+    if (e->has_nem) memcpy(e->nsnap, e->nem, e->n_labels * 2 * 8);
     if (e->has_mot) {
         memcpy(e->prev_labels, e->labels, e->n * 4);
         memcpy(e->prev_act, e->act, e->n * 4);
@@ -818,6 +887,16 @@ static void one_pass(emu_sim *e, int colour) {
                         m[6] += s * ux * uy;
                         m[7] += s * ux * uz;
                         m[8] += s * uy * uz;
+                    }
+                }
+                if (e->has_nem) {
+                    if (old != 0) {
+                        e->nem[2 * old] -= e->nq[2 * t];
+                        e->nem[2 * old + 1] -= e->nq[2 * t + 1];
+                    }
+                    if (nw != 0) {
+                        e->nem[2 * nw] += e->nq[2 * t];
+                        e->nem[2 * nw + 1] += e->nq[2 * t + 1];
                     }
                 }
                 e->accepted += 1;
@@ -1043,14 +1122,42 @@ void emu_cell_types(const emu_sim *e, uint8_t *out) { memcpy(out, e->ctype, e->n
 
 /* Words per label: volume, surface, then the nine moment sums when kept. */
 uint32_t emu_cell_state(const emu_sim *e, int64_t *out) {
-    uint32_t words = e->tracks_moments ? 11 : 2;
+    uint32_t words = e->tracks_moments ? (e->has_nem ? 13 : 11) : 2;
     if (!out) return words;
     for (size_t l = 0; l < e->n_labels; ++l) {
         out[words * l] = e->vol[l];
         out[words * l + 1] = e->surf[l];
-        if (words == 11) memcpy(out + words * l + 2, e->mom + 9 * l, 9 * 8);
+        if (words >= 11) memcpy(out + words * l + 2, e->mom + 9 * l, 9 * 8);
+        if (words == 13) memcpy(out + words * l + 11, e->nem + 2 * l, 2 * 8);
     }
     return words;
+}
+
+/* The nematic field, one (Q_xx, Q_xy) pair per site, as the engine holds it:
+ * each value rounded to the nearest multiple of 2^-24 (ties away from zero),
+ * so per-label sums are exact integers in any order. The coupling is on when
+ * a field is given and some type has a lambda_nematic (model.rs:120-124); it
+ * keeps the moments (model.rs:127-130), which start from the anchors of the
+ * current lattice if they were not kept before. Replacing the field resums. */
+int32_t emu_nematic_fixed(double q) { return (int32_t)llround(q * 16777216.0); }
+
+void emu_set_nematic(emu_sim *e, const double *q) {
+    free(e->nq);
+    e->nq = NULL;
+    e->has_nem = 0;
+    if (q) {
+        e->nq = malloc(e->n * 2 * 4);
+        for (size_t i = 0; i < 2 * e->n; ++i) e->nq[i] = emu_nematic_fixed(q[i]);
+        for (uint32_t t = 0; t < e->m.n_types; ++t)
+            if (e->m.types[t].lambda_nematic != 0.0) e->has_nem = 1;
+    }
+    if (e->has_nem) e->integral = 0; /* the term is real-valued */
+    if (e->has_nem && !e->tracks_moments) {
+        e->tracks_moments = 1;
+        anchor_from_first_sites(e);
+    } else {
+        sum_moments(e);
+    }
 }
 void emu_field_f32(const emu_sim *e, uint32_t species, float *out) {
     memcpy(out, e->field[species], e->n * 4);

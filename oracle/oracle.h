@@ -60,6 +60,10 @@ void serial_cell_types(const serial_sim *s, uint8_t *out);
 void serial_recount_volumes(const serial_sim *s, uint32_t *out);
 void serial_recount_surfaces(const serial_sim *s, int64_t *out);
 void serial_recount(serial_sim *s);
+void serial_set_nematic(serial_sim *s, const double *q);
+void serial_nematic_sums(const serial_sim *s, double *out);
+void serial_recounted_nematic(const serial_sim *s, double *out);
+void serial_recounted_anisotropy(const serial_sim *s, double *out);
 void serial_lengths(const serial_sim *s, double *out);
 void serial_recounted_lengths(const serial_sim *s, double *out);
 void serial_field(const serial_sim *s, uint32_t species, double *out);
@@ -95,6 +99,8 @@ void emu_volumes(const emu_sim *e, uint32_t *out);
 void emu_surfaces(const emu_sim *e, int64_t *out);
 void emu_cell_types(const emu_sim *e, uint8_t *out);
 uint32_t emu_cell_state(const emu_sim *e, int64_t *out);
+int32_t emu_nematic_fixed(double q);
+void emu_set_nematic(emu_sim *e, const double *q);
 void emu_field_f32(const emu_sim *e, uint32_t species, float *out);
 void emu_set_field(emu_sim *e, uint32_t species, const double *values);
 void emu_activity_f32(const emu_sim *e, float *out);

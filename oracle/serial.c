@@ -152,6 +152,21 @@ static void mom_covariance(const moments_t *m, double c[9]) {
     memcpy(c, cc, sizeof cc);
 }
 
+/* Moments::anisotropy (moments.rs:131-141): the in-plane anisotropy
+ * ((c_xx - c_yy) / tr, 2 c_xy / tr) with tr = c_xx + c_yy, zero for fewer
+ * than two sites or a trace at or below 1e-12. */
+static void mom_anisotropy(const moments_t *m, double *a0, double *a1) {
+    *a0 = 0.0;
+    *a1 = 0.0;
+    if (m->n < 2.0) return;
+    double c[9];
+    mom_covariance(m, c);
+    double trace = c[0] + c[4];
+    if (trace <= 1e-12) return;
+    *a0 = (c[0] - c[4]) / trace;
+    *a1 = 2.0 * c[1] / trace;
+}
+
 /* principal_axis (moments.rs:165-186): twenty rounds of power iteration from
  * (1, 0.7, 0.3), then the Rayleigh quotient, floored at zero. */
 void oracle_principal_axis(const double c[9], double axis[3], double *lambda) {
@@ -213,13 +228,15 @@ struct serial_sim {
     int64_t *surface;
     uint8_t *cell_type;
     moments_t *moments;
+    double *nematic;  /* (Q_xx, Q_xy) per site, or NULL */
+    double *nem_sum;  /* the field summed over each label's sites, 2 per label */
     double *activity;
     double *fields[GHIP_MAX_SPECIES];
     double *scratch;
     int substeps[GHIP_MAX_SPECIES];
     uint64_t rng[4];
     uint64_t mcs, accepted, attempted;
-    int tracks_moments, has_motility, has_population;
+    int tracks_moments, has_motility, has_population, has_length, has_nematic;
 };
 
 /* ORDER3 in lattice.rs:42-62: dz, then dy, then dx from -1 to 1. */
@@ -344,6 +361,7 @@ static void ensure_capacity(serial_sim *s, size_t want) {
     s->surface = realloc(s->surface, cap * sizeof(int64_t));
     s->cell_type = realloc(s->cell_type, cap);
     s->moments = realloc(s->moments, cap * sizeof(moments_t));
+    s->nem_sum = realloc(s->nem_sum, cap * 2 * sizeof(double));
     s->cap = cap;
 }
 
@@ -357,6 +375,8 @@ int serial_substeps(uint32_t species_index, const ghip_model *m) {
     double c = ceil(sp->diffusion / limit);
     return c < 1.0 ? 1 : (int)c;
 }
+
+static void count_nematic(const serial_sim *s, double *sums, size_t n_labels);
 
 /* Simulation::assemble (cpu.rs:178-217) from a label field and a type per
  * label (cpu.rs:220-224). */
@@ -378,6 +398,7 @@ serial_sim *serial_create(const ghip_model *m, const uint32_t *labels, const uin
         if (ct->division_volume > 0.0 || ct->death_rate > 0.0) s->has_population = 1;
     }
     s->tracks_moments = has_len;
+    s->has_length = has_len;
     s->ring_n = has_conn ? order3(s->ring, s->d <= 1) : 0;
     s->labels = malloc(s->n * sizeof(uint32_t));
     memcpy(s->labels, labels, s->n * sizeof(uint32_t));
@@ -388,6 +409,7 @@ serial_sim *serial_create(const ghip_model *m, const uint32_t *labels, const uin
     for (size_t i = 0; i < s->n; ++i) s->volume[s->labels[i]] += 1;
     count_surface(s, s->surface, n_labels);
     count_moments(s, s->moments, n_labels);
+    count_nematic(s, s->nem_sum, n_labels);
     s->activity = calloc(s->n, sizeof(double));
     for (uint32_t k = 0; k < m->n_species; ++k) {
         s->fields[k] = malloc(s->n * sizeof(double));
@@ -407,6 +429,8 @@ void serial_destroy(serial_sim *s) {
     free(s->surface);
     free(s->cell_type);
     free(s->moments);
+    free(s->nematic);
+    free(s->nem_sum);
     free(s->activity);
     for (int k = 0; k < GHIP_MAX_SPECIES; ++k) free(s->fields[k]);
     free(s->scratch);
@@ -451,6 +475,39 @@ static double length_term(const serial_sim *s, uint32_t label, const double site
     return spec->lambda_length * (a * a - b * b);
 }
 
+/* count_nematic (cpu.rs:24-34): the field summed over each label's sites. */
+static void count_nematic(const serial_sim *s, double *sums, size_t n_labels) {
+    memset(sums, 0, n_labels * 2 * sizeof(double));
+    if (!s->nematic) return;
+    for (size_t i = 0; i < s->n; ++i) {
+        uint32_t l = s->labels[i];
+        sums[2 * l] += s->nematic[2 * i];
+        sums[2 * l + 1] += s->nematic[2 * i + 1];
+    }
+}
+
+/* nematic_term (cpu.rs:315-332): the change in -lambda F . a of `label` when
+ * the site at `index` joins (sign +1) or leaves (sign -1). */
+static double nematic_term(const serial_sim *s, uint32_t label, size_t index, const double site[3],
+                           double sign) {
+    if (label == 0) return 0.0;
+    const ghip_cell_type *spec = &s->m.types[type_of(s, label)];
+    if (spec->lambda_nematic == 0.0) return 0.0;
+    const moments_t *mom = &s->moments[label];
+    double ext[3], un[3];
+    extent_of(s, ext);
+    mom_unwrap(mom, site[0], site[1], site[2], ext, un);
+    double a0, a1, b0, b1;
+    mom_anisotropy(mom, &a0, &a1);
+    moments_t with = mom_with(mom, un, sign);
+    mom_anisotropy(&with, &b0, &b1);
+    const double *f = s->nem_sum + 2 * label;
+    const double *q = s->nematic + 2 * index;
+    double before = f[0] * a0 + f[1] * a1;
+    double after = (f[0] + sign * q[0]) * b0 + (f[1] + sign * q[1]) * b1;
+    return -spec->lambda_nematic * (after - before);
+}
+
 /* delta_energy (cpu.rs:242-288). */
 double serial_delta_energy(const serial_sim *s, size_t target, uint32_t new_label) {
     uint32_t old = s->labels[target];
@@ -481,6 +538,10 @@ double serial_delta_energy(const serial_sim *s, size_t target, uint32_t new_labe
         double site[3] = {(double)tx, (double)ty, (double)tz};
         delta += length_term(s, old, site, -1.0);
         delta += length_term(s, new_label, site, 1.0);
+        if (s->has_nematic) {
+            delta += nematic_term(s, old, target, site, -1.0);
+            delta += nematic_term(s, new_label, target, site, 1.0);
+        }
     }
     delta += surface_term(s, old, 2 * like_old - bonds);
     delta += surface_term(s, new_label, bonds - 2 * like_new);
@@ -622,6 +683,14 @@ int serial_attempt(serial_sim *s) {
         mom_remove(&s->moments[old], un);
         mom_unwrap(&s->moments[nl], site[0], site[1], site[2], ext, un);
         mom_add(&s->moments[nl], un);
+    }
+    if (s->has_nematic) {
+        /* cpu.rs:488-495 */
+        const double *q = s->nematic + 2 * target;
+        s->nem_sum[2 * old] -= q[0];
+        s->nem_sum[2 * old + 1] -= q[1];
+        s->nem_sum[2 * nl] += q[0];
+        s->nem_sum[2 * nl + 1] += q[1];
     }
     if (nl != 0) {
         const ghip_cell_type *spec = &s->m.types[type_of(s, nl)];
@@ -774,6 +843,7 @@ void serial_divide_and_die(serial_sim *s, size_t *divided_out, size_t *died_out)
     if (divided > 0 || died > 0) {
         count_surface(s, s->surface, s->n_labels);
         count_moments(s, s->moments, s->n_labels);
+        if (s->has_nematic) count_nematic(s, s->nem_sum, s->n_labels); /* cpu.rs:652 */
     }
     if (divided_out) *divided_out = divided;
     if (died_out) *died_out = died;
@@ -795,7 +865,7 @@ void serial_step(serial_sim *s, uint64_t steps) {
     }
 }
 
-/* energy (cpu.rs:545-581), nematic term aside. */
+/* energy (cpu.rs:545-581). */
 double serial_energy(const serial_sim *s) {
     double contact = 0.0;
     for (size_t i = 0; i < s->n; ++i) {
@@ -811,7 +881,7 @@ double serial_energy(const serial_sim *s) {
     contact /= 2.0;
     int64_t *counted = malloc(s->n_labels * sizeof(int64_t));
     count_surface(s, counted, s->n_labels);
-    double volume = 0.0, surface = 0.0, length = 0.0;
+    double volume = 0.0, surface = 0.0, length = 0.0, nematic = 0.0;
     for (size_t l = 1; l < s->n_labels; ++l) {
         const ghip_cell_type *spec = &s->m.types[s->cell_type[l]];
         double dv = (double)s->volume[l] - spec->target_volume;
@@ -822,9 +892,15 @@ double serial_energy(const serial_sim *s) {
         }
         double ds = (double)counted[l] - spec->target_surface;
         surface += spec->lambda_surface * (ds * ds);
+        if (s->has_nematic && spec->lambda_nematic != 0.0) {
+            double a0, a1;
+            mom_anisotropy(&s->moments[l], &a0, &a1);
+            const double *f = s->nem_sum + 2 * l;
+            nematic -= spec->lambda_nematic * (f[0] * a0 + f[1] * a1);
+        }
     }
     free(counted);
-    return contact + volume + surface + length;
+    return contact + volume + surface + length + nematic;
 }
 
 size_t serial_n_labels(const serial_sim *s) { return s->n_labels; }
@@ -857,6 +933,36 @@ void serial_recount(serial_sim *s) {
     serial_recount_volumes(s, s->volume);
     count_surface(s, s->surface, s->n_labels);
     count_moments(s, s->moments, s->n_labels);
+    count_nematic(s, s->nem_sum, s->n_labels);
+}
+
+/* Model::nematic for this run (2 values per site), or NULL to drop it; the
+ * coupling is on when some type has a lambda_nematic (model.rs:120-124),
+ * and the moments are kept for it (model.rs:127-130). */
+void serial_set_nematic(serial_sim *s, const double *q) {
+    free(s->nematic);
+    s->nematic = NULL;
+    s->has_nematic = 0;
+    if (q) {
+        s->nematic = malloc(s->n * 2 * sizeof(double));
+        memcpy(s->nematic, q, s->n * 2 * sizeof(double));
+        for (uint32_t t = 0; t < s->m.n_types; ++t)
+            if (s->m.types[t].lambda_nematic != 0.0) s->has_nematic = 1;
+    }
+    s->tracks_moments = s->has_length || s->has_nematic;
+    count_nematic(s, s->nem_sum, s->n_labels);
+}
+void serial_nematic_sums(const serial_sim *s, double *out) {
+    memcpy(out, s->nem_sum, s->n_labels * 2 * sizeof(double));
+}
+void serial_recounted_nematic(const serial_sim *s, double *out) { count_nematic(s, out, s->n_labels); }
+/* Each label's anisotropy (moments.rs:131-141) from moments recounted from
+ * the lattice (count_moments, cpu.rs:11-21), 2 values per label. */
+void serial_recounted_anisotropy(const serial_sim *s, double *out) {
+    moments_t *m = malloc(s->n_labels * sizeof(moments_t));
+    count_moments(s, m, s->n_labels);
+    for (size_t l = 0; l < s->n_labels; ++l) mom_anisotropy(&m[l], &out[2 * l], &out[2 * l + 1]);
+    free(m);
 }
 void serial_lengths(const serial_sim *s, double *out) {
     for (size_t l = 0; l < s->n_labels; ++l) out[l] = mom_length(&s->moments[l]);
@@ -895,6 +1001,13 @@ serial_sim *serial_clone(const serial_sim *s) {
     memcpy(c->cell_type, s->cell_type, s->cap);
     c->moments = malloc(s->cap * sizeof(moments_t));
     memcpy(c->moments, s->moments, s->cap * sizeof(moments_t));
+    c->nem_sum = malloc(s->cap * 2 * sizeof(double));
+    memcpy(c->nem_sum, s->nem_sum, s->cap * 2 * sizeof(double));
+    c->nematic = NULL;
+    if (s->nematic) {
+        c->nematic = malloc(s->n * 2 * sizeof(double));
+        memcpy(c->nematic, s->nematic, s->n * 2 * sizeof(double));
+    }
     c->activity = malloc(s->n * sizeof(double));
     memcpy(c->activity, s->activity, s->n * sizeof(double));
     for (int k = 0; k < GHIP_MAX_SPECIES; ++k) {
@@ -921,6 +1034,13 @@ void serial_apply_copy(serial_sim *s, size_t target, uint32_t new_label) {
     mom_remove(&s->moments[old], un);
     mom_unwrap(&s->moments[new_label], (double)tx, (double)ty, (double)tz, ext, un);
     mom_add(&s->moments[new_label], un);
+    if (s->nematic) {
+        const double *q = s->nematic + 2 * target;
+        s->nem_sum[2 * old] -= q[0];
+        s->nem_sum[2 * old + 1] -= q[1];
+        s->nem_sum[2 * new_label] += q[0];
+        s->nem_sum[2 * new_label + 1] += q[1];
+    }
     s->labels[target] = new_label;
     s->volume[old] -= 1;
     s->volume[new_label] += 1;

@@ -160,7 +160,31 @@ def case(name):
         m = ref_model(16, 16, 16, seed=62, order=3, division_volume=72.0, death_rate=0.01)
         m.types[1].target_volume = 100.0
         return (m, *tiled(m, side=4), None, None, 10)
+    if name == "nematic":
+        # nematic.rs:69-113's model: random field, lambda_N 3
+        m = ref_model(64, 64, seed=71, lambda_nematic=3.0)
+        m.nematic = random_pairs(9, m.sites)
+        return (m, *tiled(m), None, None, 20)
+    if name == "nematic_division":
+        m = ref_model(64, 64, seed=72, lambda_nematic=2.0, division_volume=72.0, death_rate=0.005)
+        m.types[1].target_volume = 100.0
+        m.nematic = random_pairs(4, m.sites)
+        return (m, *tiled(m, side=6, nx=6, ny=6), None, None, 30)
+    if name == "nematic3d":
+        m = ref_model(16, 16, 16, seed=73, order=3, lambda_nematic=4.0)
+        m.types[1].target_volume = 512.0
+        m.nematic = random_pairs(6, m.sites)
+        return (m, *tiled(m), None, None, 5)
     raise KeyError(name)
+
+
+def random_pairs(seed, sites):
+    """nematic.rs:19-24: (next_f64() - 0.5, next_f64() - 0.5) per site from
+    xoshiro256++ seeded with `seed`."""
+    import oracle
+
+    rng = oracle.Xoshiro(seed)
+    return np.array([[rng.next_f64() - 0.5, rng.next_f64() - 0.5] for _ in range(sites)])
 
 
 PARITY_CASES = ["plain128", "surface256", "surface1024", "fast_medium_flat", "fast_medium_typed",
@@ -168,4 +192,4 @@ PARITY_CASES = ["plain128", "surface256", "surface1024", "fast_medium_flat", "fa
                 "von_neumann", "float_energies", "volume3d_o2", "volume3d_o3", "volume3d_o1",
                 "connected", "connected3d", "chemotaxis_ramp", "secreting", "length",
                 "motility", "drift", "death", "division", "infection128", "motility3d",
-                "length3d", "division3d"]
+                "length3d", "division3d", "nematic", "nematic_division", "nematic3d"]

@@ -65,9 +65,10 @@ def test_odd_lattices_and_unpriced_terms_are_refused(lib):
     assert status == engine.GHIP_ODD_LATTICE and not h.value
     assert b"even" in lib.ghip_last_error()
     m, labels, types = plain_monolayer(64, 1)
-    m.has_nematic = True
-    status, h = _create(lib, m, labels, types)
-    assert status == engine.GHIP_UNSUPPORTED and b"nematic" in lib.ghip_last_error()
+    m.types[1].lambda_nematic = 1.0
+    m.nematic = np.zeros((64 * 64, 2))
+    status, h = _create(lib, m, labels, types)  # the field goes through ghip_create_nematic
+    assert status == engine.GHIP_BAD_ARG and b"nematic" in lib.ghip_last_error()
     m, labels, types = plain_monolayer(64, 1)
     bad = types.copy()
     bad[3] = 7

@@ -18,7 +18,8 @@ pytestmark = pytest.mark.gpu
 BATCH_CASES = ["plain128", "fast_medium_typed", "medium_gaps", "rect64x32", "von_neumann",
                "float_energies", "volume3d_o2", "volume3d_o3", "volume3d_o1", "connected",
                "connected3d", "chemotaxis_ramp", "secreting", "length", "motility", "drift",
-               "death", "division", "infection128", "motility3d", "length3d", "division3d"]
+               "death", "division", "infection128", "motility3d", "length3d", "division3d",
+               "nematic", "nematic_division", "nematic3d"]
 
 
 def assert_same(name, m, gpu, emu):
@@ -82,13 +83,15 @@ def test_every_tile_candidate_gives_the_same_lattice(engine, oracle_lib, candida
 def test_a_lattice_stepped_alone_between_batch_steps(engine, oracle_lib):
     from glazier_amd import HipBatch
 
-    runs = build(["infection128", "motility", "connected3d"])
+    # plain128 steps alone on the resident single-workgroup pass
+    runs = build(["infection128", "motility", "connected3d", "plain128"])
     batch = HipBatch([r[2] for r in runs])
     batch.step(7)
     runs[1][2].step(2)
+    runs[3][2].step(2)
     batch.step(3)
     for i, (name, m, gpu, emu) in enumerate(runs):
-        emu.step(12 if i == 1 else 10)
+        emu.step(12 if i in (1, 3) else 10)
         assert_same(name, m, gpu, emu)
     batch.close()
 

@@ -59,3 +59,21 @@ def test_an_infection_run_keeps_its_field_and_counts(tmp_path):
     assert (out / "field_virus_00015.npy").exists()
     labels = np.load(out / "labels_00030.npy")
     assert summary["n_cells"] == np.unique(labels[labels > 0]).size
+
+
+def test_a_nematic_blueprint_runs_with_its_field(tmp_path):
+    # monolayer-nematic.json names its field beside itself (blueprint.rs:347-358)
+    raw = json.loads((GOLDEN / "monolayer-nematic.json").read_text())
+    raw["steps"] = 40
+    raw["dump_every"] = 20
+    raw["nematic_field"] = str(GOLDEN / raw["nematic_field"])
+    path = tmp_path / "monolayer-nematic.json"
+    path.write_text(json.dumps(raw))
+    out = tmp_path / "out"
+    print("[cli] monolayer-nematic.json 40 steps", flush=True)
+    done = subprocess.run([str(BINARY), "--model", str(path), "--out", str(out), "--engine", "hip"],
+                          capture_output=True, text=True, timeout=120)
+    assert done.returncode == 0, done.stderr
+    summary = json.loads((out / "summary.json").read_text())
+    assert summary["n_cells"] == 256 and abs(summary["mean_volume"] - 64.0) < 1e-9
+    assert np.load(out / "labels_00040.npy").shape == (128, 128)
