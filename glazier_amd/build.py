@@ -17,9 +17,9 @@ HERE = Path(__file__).resolve().parent
 CSRC = HERE / "csrc"
 OUT = HERE / "libglazier_hip.so"
 CLI = HERE / "glazier"
-CLI_SOURCES = ["cli.cpp", "blueprint.cpp"]
+CLI_SOURCES = ["cli.cpp"]
 SOURCES = ["sweep.hip", "sweep_plain.hip", "fields.hip", "cells.hip", "batch.hip", "adjoint.hip",
-           "engine.cpp", "batch_host.cpp", "fit_host.cpp"]
+           "engine.cpp", "batch_host.cpp", "fit_host.cpp", "blueprint.cpp"]
 OBJ_DIR = HERE / "_build"
 FLAGS = [
     "--offload-arch=gfx950",

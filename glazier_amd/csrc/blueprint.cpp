@@ -637,7 +637,7 @@ std::string list(size_t n, F f) {
 }  // namespace
 
 // The keys of glazier/src/main.rs:62-96 (serde_json prints them sorted).
-std::string check_json(const Blueprint &bp) {
+std::string check_json(const Blueprint &bp, bool with_depth) {
     ghip_model m = bp.model();
     const size_t nt = m.n_types, ns = m.n_species;
     std::vector<std::pair<std::string, std::string>> kv;
@@ -649,6 +649,7 @@ std::string check_json(const Blueprint &bp) {
     kv.push_back({"contact", list(nt * nt, [&](size_t i) { return num(m.contact[i]); })});
     kv.push_back({"death_rate", per_type(&ghip_cell_type::death_rate)});
     kv.push_back({"decay", list(ns, [&](size_t k) { return num(m.species[k].decay); })});
+    if (with_depth) kv.push_back({"depth", std::to_string(m.depth)});
     kv.push_back({"diffusion", list(ns, [&](size_t k) { return num(m.species[k].diffusion); })});
     kv.push_back({"division_volume", per_type(&ghip_cell_type::division_volume)});
     kv.push_back({"external", list(nt, [&](size_t i) { return list(3, [&](size_t a) { return num(m.types[i].external[a]); }); })});

@@ -95,7 +95,10 @@ uint32_t tile_grid(const ghip_model &m, uint32_t side, uint32_t nx, uint32_t ny,
                    std::vector<uint32_t> *labels);
 
 // The `--check` reading (glazier/src/main.rs:59-101), as pretty JSON.
-std::string check_json(const Blueprint &bp);
+// What the reader took from a description, as `glazier --check` prints it
+// (glazier/src/main.rs:62-96); with_depth adds the lattice depth, which the
+// Python binding's parse reports beside it (glazier-py/src/lib.rs:18-150).
+std::string check_json(const Blueprint &bp, bool with_depth = false);
 
 // .npy I/O (glazier-core/src/npy.rs).
 bool npy_write_u32(const std::string &path, const uint32_t *data, uint32_t w, uint32_t h, uint32_t d,

@@ -17,6 +17,7 @@
 #include <vector>
 
 #include "../../include/glazier_hip.h"
+#include "blueprint.h"
 #include "fit.h"
 #include "ghip_device.h"
 #include "sim.h"
@@ -762,6 +763,19 @@ ghip_status ghip_field(const ghip_sim *s, uint32_t species, double *out, size_t 
     ghip_status st = ghip_field_f32(s, species, f.data(), n);
     if (st != GHIP_OK) return st;
     for (size_t i = 0; i < n; ++i) out[i] = (double)f[i];
+    return GHIP_OK;
+}
+
+ghip_status ghip_blueprint_parse(const char *text, char *out, size_t cap, size_t *needed) {
+    if (!text || !needed) return fail(GHIP_BAD_ARG, "null argument");
+    Blueprint bp;
+    std::string err;
+    if (!blueprint_from_json(text, &bp, &err)) return fail(GHIP_BAD_ARG, err);
+    const std::string json = check_json(bp, true);
+    *needed = json.size() + 1;
+    if (!out) return GHIP_OK;
+    if (cap < json.size() + 1) return fail(GHIP_BAD_ARG, "the buffer is shorter than the description");
+    std::memcpy(out, json.c_str(), json.size() + 1);
     return GHIP_OK;
 }
 

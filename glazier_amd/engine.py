@@ -64,6 +64,8 @@ def library() -> ctypes.CDLL:
                                         P(ctypes.c_uint8), ctypes.c_size_t, P(ctypes.c_double),
                                         P(ctypes.c_double), P(ctypes.c_double), P(ctypes.c_void_p)]
     lib.ghip_set_nematic_field.argtypes = [sim, P(ctypes.c_double), ctypes.c_size_t]
+    lib.ghip_blueprint_parse.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_size_t,
+                                         P(ctypes.c_size_t)]
     lib.ghip_destroy.argtypes = [sim]
     lib.ghip_step.argtypes = [sim, ctypes.c_uint64]
     lib.ghip_mcs.argtypes = [sim]

@@ -116,6 +116,14 @@ ghip_status ghip_create_nematic(const ghip_model *model, const uint32_t *labels,
                                 const uint8_t *cell_type, size_t n_labels, const double *fields,
                                 const double *activity, const double *nematic, ghip_sim **out);
 
+/* Replaces the Python binding's parse (glazier-py/src/lib.rs:18-150): reads a
+ * Blueprint description (JSON text, glazier-core/src/blueprint.rs:195-447)
+ * with this engine's reader and writes what it took as a JSON object, the
+ * keys of `glazier --check` (glazier/src/main.rs:62-96) plus "depth".
+ * *needed receives the size with the terminating zero; `out` may be NULL to
+ * ask for it. GHIP_BAD_ARG, with the reader's message, on a bad description. */
+ghip_status ghip_blueprint_parse(const char *text, char *out, size_t cap, size_t *needed);
+
 /* Replaces GpuSimulation::set_nematic_field (glazier-cuda/src/lib.rs:1199-1221):
  * `q` holds 2 * n values, n = n_sites. Fails when the model has no nematic
  * coupling. Every cell's field sum is recounted before the next step. */
