@@ -340,6 +340,82 @@ def blueprints_bench():
     return out
 
 
+# README.md:153-162 and 170-178: the reference's glazier CPU and GPU columns
+COMPARISON_PUBLISHED = {
+    "monolayer-physicell.json": {
+        "glazier_cpu": {"cells": 256, "mean_area": 64.0, "area_rms_from_target": 1.58,
+                        "mean_eccentricity": 0.397, "nematic_order": 0.064, "defects": 56,
+                        "net_charge": 0.0, "seconds": 0.146},
+        "glazier_gpu": {"cells": 256, "mean_area": 64.0, "area_rms_from_target": 2.07,
+                        "mean_eccentricity": 0.392, "nematic_order": 0.052, "defects": 50,
+                        "net_charge": 0.0, "seconds": 0.106}},
+    "monolayer-surface.json": {
+        "glazier_cpu": {"mean_eccentricity": 0.371, "area_rms_from_target": 1.61, "defects": 34},
+        "glazier_gpu": {"mean_eccentricity": 0.359, "area_rms_from_target": 1.72, "defects": 44}},
+    "monolayer.json": {
+        "glazier_cpu": {"mean_eccentricity": 0.394},
+        "glazier_gpu": {"mean_eccentricity": 0.394}},
+}
+
+
+def comparison_bench():
+    """The engine comparison (bench/engines.py, README.md:147-182) for this
+    engine: each blueprint run for its 200 steps through the `glazier`
+    binary, the final label field measured as the reference measures it
+    (glazier_amd.director, coarse-grain sigma 4), the binary's own seconds."""
+    import subprocess
+    import tempfile
+
+    import numpy as np
+
+    from glazier_amd import director
+    from glazier_amd.blueprint import Blueprint
+
+    root = Path(__file__).resolve().parent
+    binary = root / "glazier_amd" / "glazier"
+    out = {}
+    for name, published in COMPARISON_PUBLISHED.items():
+        path = root / "tests" / "golden" / "blueprints" / name
+        bp = Blueprint.load(path)
+        with tempfile.TemporaryDirectory() as tmp:
+            done = subprocess.run([str(binary), "--model", str(path), "--out", tmp, "--engine", "hip"],
+                                  capture_output=True, text=True, timeout=300)
+            if done.returncode != 0:
+                out[name] = {"error": done.stderr.strip()[-300:]}
+                continue
+            summary = json.loads((Path(tmp) / "summary.json").read_text())
+            labels = np.load(Path(tmp) / f"labels_{bp.steps:05d}.npy")
+        row = director.measure(labels, bp.types[0].target_volume, 4.0)
+        row["seconds"] = summary["seconds"]
+        out[name] = {"hip": row, "reference_published": published}
+    return out
+
+
+def cpu_baseline_comparison():
+    """The serial oracle's column of the engine comparison: the same
+    blueprints and steps on one thread, measured the same way."""
+    import numpy as np
+
+    import oracle
+    from glazier_amd import director
+    from glazier_amd.blueprint import Blueprint
+
+    root = Path(__file__).resolve().parent
+    out = {}
+    for name in COMPARISON_PUBLISHED:
+        bp = Blueprint.load(root / "tests" / "golden" / "blueprints" / name)
+        model, labels, types = bp.simulation()
+        sim = oracle.SerialSimulation(model, labels, types)
+        t0 = time.perf_counter()
+        sim.step(bp.steps)
+        el = time.perf_counter() - t0
+        row = director.measure(sim.labels().reshape(model.shape()), bp.types[0].target_volume, 4.0)
+        row["seconds"] = el
+        out[name] = row
+        sim.close()
+    return out
+
+
 def counter_per_dispatch(args, counter: str, kernel: str):
     """Run this script under `rocprofv3 --pmc <counter>` in a child process
     (the program directly after `--`) and return the counter's mean value per
@@ -508,6 +584,7 @@ def main() -> int:
     if not args.no_configs:
         line["configs"] = configs_bench()
         line["blueprints"] = blueprints_bench()
+        line["engine_comparison"] = comparison_bench()
     if not args.no_ensemble:
         line["ensemble"] = ensemble_bench(args.ensemble_mcs)
     leg = None
@@ -520,6 +597,7 @@ def main() -> int:
             line["cpu_baseline"]["adjoint"] = cpu_baseline_adjoint(leg)
         if not args.no_configs:
             line["cpu_baseline"]["configs"] = cpu_baseline_configs()
+            line["cpu_baseline"]["engine_comparison"] = cpu_baseline_comparison()
     print(json.dumps(line))
     return 0
 
