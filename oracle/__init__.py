@@ -87,6 +87,7 @@ def library() -> ctypes.CDLL:
         "serial_clone": ([v], v),
         "serial_apply_copy": ([v, sz, u32], None),
         "serial_set_nematic": ([v, P(f64)], None),
+        "serial_decay_activity": ([v], None),
         "serial_nematic_sums": ([v, P(f64)], None),
         "serial_recounted_nematic": ([v, P(f64)], None),
         "serial_recounted_anisotropy": ([v, P(f64)], None),
@@ -303,6 +304,10 @@ class SerialSimulation(_Base):
 
     def locally_connected(self, target: int, label: int) -> bool:
         return bool(library().serial_locally_connected(self._h, target, label))
+
+    def decay_activity(self) -> None:
+        """One Activity::decay (motility.rs:38-45)."""
+        library().serial_decay_activity(self._h)
 
     def set_nematic_field(self, q: np.ndarray) -> None:
         """Model::nematic replaced (2 values per site); the sums are recounted."""

@@ -60,6 +60,7 @@ void serial_cell_types(const serial_sim *s, uint8_t *out);
 void serial_recount_volumes(const serial_sim *s, uint32_t *out);
 void serial_recount_surfaces(const serial_sim *s, int64_t *out);
 void serial_recount(serial_sim *s);
+void serial_decay_activity(serial_sim *s);
 void serial_set_nematic(serial_sim *s, const double *q);
 void serial_nematic_sums(const serial_sim *s, double *out);
 void serial_recounted_nematic(const serial_sim *s, double *out);

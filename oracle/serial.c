@@ -985,6 +985,11 @@ void serial_activity(const serial_sim *s, double *out) {
 void serial_set_activity(serial_sim *s, const double *values) {
     memcpy(s->activity, values, s->n * sizeof(double));
 }
+/* Activity::decay (motility.rs:38-45): every remembered site loses one. */
+void serial_decay_activity(serial_sim *s) {
+    for (size_t i = 0; i < s->n; ++i)
+        if (s->activity[i] > 0.0) s->activity[i] -= 1.0;
+}
 
 /* A deep copy, for what-if checks (the tests clone a Simulation the same
  * way, e.g. glazier-tests/tests/length.rs:50-64). */

@@ -145,6 +145,64 @@ def test_length_energy_changes_match_exact_eigenvalues(oracle_lib):
     assert checked > 30
 
 
+def label_length(labels, m, label):
+    """Major axis of one label from its sites, unwrapped about its first site
+    by minimum image (valid while a cell spans under half of every side)."""
+    d, h, w = m.depth, m.height, m.width
+    idx = np.nonzero(labels == label)[0]
+    if idx.size < 2:
+        return 0.0
+    c = np.stack([idx % w, (idx // w) % h, idx // (w * h)], axis=1).astype(np.float64)
+    delta = c - c[0]
+    for ax, span in enumerate((w, h, d)):
+        if span > 1:
+            delta[:, ax] -= span * np.round(delta[:, ax] / span)
+    cov = np.cov(delta.T, bias=True)
+    return 4.0 * np.sqrt(max(np.linalg.eigvalsh(cov)[-1], 0.0))
+
+
+def test_length_energy_changes_in_a_volume_match_exact_eigenvalues(oracle_lib):
+    # the 3D length term (Jacobi on the integer covariance) against numpy's
+    # eigensolver on the cells' sites, as the planar test above; 32^3 with
+    # cubes of 8, so every cell spans well under half of each side
+    import copy
+
+    from tests.test_oracle_terms import base
+
+    m = base(61)
+    m.types[1].target_length, m.types[1].lambda_length = 14.0, 2.0
+    emu = oracle.EmuSimulation(m, *tiled(m))
+    emu.step(3)
+    lab = emu.labels()
+    m0 = copy.deepcopy(m)
+    m0.types[1].lambda_length = 0.0
+    ser = oracle.SerialSimulation(m0, lab, emu.cell_types())
+    lam, lt = m.types[1].lambda_length, m.types[1].target_length
+    rng = np.random.default_rng(5)
+    n = m.width
+    checked = 0
+    for _ in range(400):
+        t = int(rng.integers(lab.size))
+        x, y, z = t % n, (t // n) % n, t // (n * n)
+        dx, dy, dz = rng.integers(-1, 2, size=3)
+        s_ = int((((z + dz) % n) * n + (y + dy) % n) * n + (x + dx) % n)
+        old, new = int(lab[t]), int(lab[s_])
+        if old == new:
+            continue
+        after_lab = lab.copy()
+        after_lab[t] = new
+        want = ser.delta_energy(t, new)
+        for c in (old, new):
+            if c != 0:
+                want += lam * ((label_length(after_lab, m, c) - lt) ** 2 - (label_length(lab, m, c) - lt) ** 2)
+        got = emu.delta_energy(t, s_)
+        assert abs(got - want) <= 2e-5 * max(1.0, abs(want)), (got, want)
+        checked += 1
+        if checked == 60:
+            break
+    assert checked == 60
+
+
 def test_the_emulation_is_deterministic(oracle_lib):
     m, labels, types, *_ = case("infection128")
     a = oracle.EmuSimulation(m, labels, types)
