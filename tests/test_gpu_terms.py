@@ -173,3 +173,56 @@ def test_a_length_target_stretches_a_cell_through_the_depth_on_the_device(engine
         g.close()
     print(f"[terms] device length in a volume: {axes[0]:.2f} against {axes[1]:.2f}", flush=True)
     assert axes[0] > axes[1] + 1.5, axes
+
+
+def test_the_device_keeps_cells_whole_where_nothing_else_does(engine):
+    # connectivity.rs:119-152
+    from tests.test_oracle_terms import conn_model, pieces
+
+    loose_m, whole_m = conn_model(False, 40.0), conn_model(True, 40.0)
+    labels, types = tiled(whole_m)
+    cells = int(labels.max())
+    loose = hip(loose_m, labels, types)
+    whole = hip(whole_m, labels, types)
+    loose.step(200)
+    whole.step(200)
+    torn, kept = pieces(loose.labels(), loose_m), pieces(whole.labels(), whole_m)
+    print(f"[terms] device pieces: unconstrained {torn}, constrained {kept} of {cells}", flush=True)
+    assert torn > cells, "the unconstrained device sheet stayed whole"
+    assert kept == cells, "a constrained cell broke on the device"
+    loose.close()
+    whole.close()
+
+
+def test_the_device_and_the_serial_engine_agree_on_a_constrained_sheet(engine, oracle_lib):
+    # connectivity.rs:154-187
+    import oracle
+    from tests.test_oracle_terms import conn_model
+
+    m = conn_model(True, 20.0)
+    labels, types = tiled(m)
+    cpu = oracle.SerialSimulation(m, labels, types)
+    cpu.step(150)
+    gpu = hip(m, labels, types)
+    gpu.step(150)
+    vd, vc = gpu.volumes()[1:], cpu.volumes()[1:]
+    assert int(vd.sum()) == int(vc.sum()), "a confluent sheet conserves its sites on either engine"
+    spread = lambda v: float(np.sqrt(((v.astype(np.float64) - 64.0) ** 2).mean()))
+    sc, sg = spread(vc), spread(vd)
+    assert abs(sc - sg) / max(sc, sg) < 0.3, (sc, sg)
+    gpu.close()
+
+
+def test_a_molecule_description_runs_on_the_device_as_the_matrix_it_folds_to(engine):
+    # adhesion.rs:84-103 on the device: the same lattice either way
+    from glazier_amd.blueprint import Blueprint
+    from tests.test_oracle_terms import adhesion_text, written_out_text
+
+    runs = []
+    for text in (adhesion_text(4.0, 1.0), written_out_text(6.0)):
+        m, labels, types = Blueprint.from_json(text).simulation()
+        g = hip(m, labels, types)
+        g.step(150)
+        runs.append(g.labels())
+        g.close()
+    assert np.array_equal(runs[0], runs[1])
