@@ -8,6 +8,7 @@ and Blueprint reader, and the ctypes binding of the engine.
 
 from .model import CellType, Model, Species, default_cell_types, tiled_labels
 from .engine import GhipError, HipBatch, HipSimulation
+from .director import mid_plane
 
 __all__ = [
     "CellType",
@@ -17,5 +18,6 @@ __all__ = [
     "HipBatch",
     "HipSimulation",
     "default_cell_types",
+    "mid_plane",
     "tiled_labels",
 ]

@@ -115,6 +115,11 @@ class Blueprint:
     base: Path = Path(".")
 
     @property
+    def dimensions(self) -> int:
+        """Two for a plane, three for a volume (python/glazier/blueprint.py:74-77)."""
+        return 3 if self.depth > 1 else 2
+
+    @property
     def type_names(self) -> List[str]:
         return [MEDIUM] + [t.name for t in self.types]
 

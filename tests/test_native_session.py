@@ -85,3 +85,51 @@ def test_a_session_steps_reads_back_and_takes_a_new_field(engine):
     with pytest.raises(ValueError, match="expected 32768 numbers"):
         s.set_nematic_field(flat[:-2])
     s.close()
+
+
+# tests/test_volume_measurement.py of the reference: a volume is measured on
+# its middle plane
+
+@pytest.fixture(scope="module")
+def volume_field():
+    from glazier_amd import native
+
+    text = (GOLDEN / "volume.json").read_text()
+    labels, width, height, depth = native.run(text, 5)
+    return np.asarray(labels, dtype=np.uint32).reshape(depth, height, width)
+
+
+@pytest.mark.gpu
+def test_a_volume_comes_back_three_dimensional(engine, volume_field):
+    bp = Blueprint.load(GOLDEN / "volume.json")
+    assert bp.dimensions == 3
+    assert volume_field.shape == (bp.depth, bp.height, bp.width)
+
+
+@pytest.mark.gpu
+def test_a_slice_is_one_layer_of_it(engine, volume_field):
+    from glazier_amd import mid_plane
+
+    plane = mid_plane(volume_field)
+    assert plane.shape == volume_field.shape[1:]
+    assert np.array_equal(plane, volume_field[volume_field.shape[0] // 2])
+
+
+def test_a_plane_is_returned_unchanged():
+    from glazier_amd import mid_plane
+
+    flat = np.arange(12, dtype=np.uint32).reshape(3, 4)
+    assert np.array_equal(mid_plane(flat), flat)
+
+
+@pytest.mark.gpu
+def test_the_measurements_run_on_a_sliced_volume(engine, volume_field):
+    from glazier_amd import director, mid_plane
+
+    plane = mid_plane(volume_field)
+    shapes = director.cell_shapes(plane)
+    assert shapes, "the slice held no cells"
+    q = director.coarse_grain(director.q_field(plane, shapes), 4.0)
+    charge = sum(c for _, _, c in director.defect_charges(director.director_angle(q)))
+    # a slice of a periodic volume is periodic in both axes, so its charges cancel
+    assert abs(charge) < 1e-9
