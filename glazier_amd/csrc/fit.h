@@ -39,9 +39,24 @@ hipError_t launch_fit_substep_adjoint(const FitGrid &g, const FitCoef &c, uint32
 hipError_t launch_fit_exchange_adjoint(const FitGrid &g, const FitCoef &c, uint32_t species,
                                        const uint8_t *types, const float *u_pre, float *lam, double *part,
                                        hipStream_t st);
-// Workgroups of a site launch, and the reduction of the partials.
+// One launch per whole step on a 2D lattice (substeps and exchange, or the
+// adjoint of both), for K = the largest substep count when fit_fused_fits.
+bool fit_fused_fits(const FitGrid &g, int K);
+// `steps` (1..fit_steps_per_launch()) forward steps in one launch: the
+// owners of step q at types + q * n; the state after step q at out_base +
+// q * out_stride, the last at out_last.
+int fit_steps_per_launch();
+hipError_t launch_fit_steps_fused(const FitGrid &g, const FitCoef &c, uint32_t species, int K,
+                                  const uint8_t *types, int steps, const float *in, float *out_base,
+                                  size_t out_stride, float *out_last, hipStream_t st);
+hipError_t launch_fit_step_adjoint_fused(const FitGrid &g, const FitCoef &c, uint32_t species, int K,
+                                         const uint8_t *types, const float *u_start, const float *lam_out,
+                                         float *lam_in, double *part, hipStream_t st);
+// Workgroups per species of a per-substep launch and of a whole-step launch,
+// and the reduction of the partials of `blocks` workgroups per species.
 uint32_t fit_blocks(const FitGrid &g);
-hipError_t launch_fit_reduce(const FitGrid &g, uint32_t species, const double *part, double *grad,
+uint32_t fit_tiles(const FitGrid &g);
+hipError_t launch_fit_reduce(uint32_t species, uint32_t blocks, const double *part, double *grad,
                              hipStream_t st);
 
 }  // namespace ghip

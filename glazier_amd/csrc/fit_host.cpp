@@ -8,9 +8,16 @@
 // exchange, then the transposed substeps. Memory: checkpoints (steps / K),
 // K step starts and substeps + 1 substep states, each n_species * n_sites
 // f32, plus the one-byte owner maps of the trajectory.
+//
+// On a 2D lattice whose substep count fits (fit_fused_fits) a step is one
+// launch each way, the substep states kept in LDS; elsewhere, or with
+// GLAZIER_FIT_PER_SUBSTEP=1 in the environment at creation, one launch per
+// substep and exchange. Both give the same fields and adjoint fields bit for
+// bit.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -39,8 +46,10 @@ struct ghip_fit {
     hipStream_t own = nullptr;
     FitCoef coef{};
     bool have_forward = false;
+    bool fused = false;          // one launch per step (2D)
 
     size_t field_words() const { return (size_t)S * g.n; }
+    uint32_t part_blocks() const { return fused ? fit_tiles(g) : fit_blocks(g); }
 
     ~ghip_fit() {
         if (own) (void)hipStreamSynchronize(own);
@@ -69,7 +78,8 @@ FitCoef coef_of(const ghip_fit *f, const ghip_field_params &p) {
     return c;
 }
 
-// One whole step m from `src` into `dst` (neither a work buffer).
+// One whole step m from `src` into `dst` (neither a work buffer), one
+// launch per substep and exchange.
 ghip_status forward_step(ghip_fit *f, const float *src, float *dst, uint64_t m, hipStream_t st) {
     const float *a = src;
     int w = 0;
@@ -79,6 +89,33 @@ ghip_status forward_step(ghip_fit *f, const float *src, float *dst, uint64_t m, 
         w ^= 1;
     }
     HIP_TRY(launch_fit_exchange(f->g, f->coef, f->S, f->types + m * f->g.n, a, dst, st));
+    return GHIP_OK;
+}
+
+// Steps m0 .. m0 + count - 1 from `src`: the state after step m0 + q goes
+// to out_base + q * fw, the last one to `last`. No destination is `src` or a
+// work buffer.
+ghip_status forward_steps(ghip_fit *f, const float *src, uint64_t m0, uint64_t count, float *out_base,
+                          float *last, hipStream_t st) {
+    const size_t fw = f->field_words();
+    auto dst_of = [&](uint64_t q) { return q + 1 == count ? last : out_base + q * fw; };
+    if (!f->fused) {
+        const float *cur = src;
+        for (uint64_t q = 0; q < count; ++q) {
+            ghip_status s = forward_step(f, cur, dst_of(q), m0 + q, st);
+            if (s != GHIP_OK) return s;
+            cur = dst_of(q);
+        }
+        return GHIP_OK;
+    }
+    const uint64_t per = (uint64_t)fit_steps_per_launch();
+    const float *cur = src;
+    for (uint64_t q = 0; q < count; q += per) {
+        const uint64_t n = std::min(per, count - q);
+        HIP_TRY(launch_fit_steps_fused(f->g, f->coef, f->S, f->max_sub, f->types + (m0 + q) * f->g.n, (int)n,
+                                       cur, out_base + q * fw, fw, dst_of(q + n - 1), st));
+        cur = dst_of(q + n - 1);
+    }
     return GHIP_OK;
 }
 
@@ -145,18 +182,20 @@ ghip_status ghip_fit_create(const ghip_sim *s, uint32_t checkpoint_every, ghip_f
     f->S = s->m.n_species;
     f->steps = s->traj_len;
     f->K = (uint32_t)std::min<uint64_t>(checkpoint_every, f->steps);
+    const char *per_substep = std::getenv("GLAZIER_FIT_PER_SUBSTEP");
+    f->fused = fit_fused_fits(f->g, f->max_sub) && !(per_substep && per_substep[0] == '1');
     const size_t fw = f->field_words();
     const uint64_t n_ckpt = (f->steps + f->K - 1) / f->K;
     FIT_TRY(hipStreamCreateWithFlags(&f->own, hipStreamNonBlocking));
     FIT_TRY(hipMalloc(&f->types, f->steps * s->N));
     FIT_TRY(hipMalloc(&f->ckpt, n_ckpt * fw * 4));
     FIT_TRY(hipMalloc(&f->seg, (size_t)f->K * fw * 4));
-    FIT_TRY(hipMalloc(&f->sub, (size_t)f->max_sub * fw * 4));
+    if (!f->fused) FIT_TRY(hipMalloc(&f->sub, (size_t)f->max_sub * fw * 4));
     for (int i = 0; i < 2; ++i) {
         FIT_TRY(hipMalloc(&f->work[i], fw * 4));
         FIT_TRY(hipMalloc(&f->lam[i], fw * 4));
     }
-    FIT_TRY(hipMalloc(&f->part, (size_t)f->S * fit_blocks(f->g) * kPartWords * sizeof(double)));
+    FIT_TRY(hipMalloc(&f->part, (size_t)f->S * f->part_blocks() * kPartWords * sizeof(double)));
     FIT_TRY(hipMalloc(&f->grad, (size_t)f->S * kPartWords * sizeof(double)));
     FIT_TRY(hipStreamSynchronize(s->stream));
     FIT_TRY(hipMemcpy(f->types, s->traj, f->steps * s->N, hipMemcpyDeviceToDevice));
@@ -192,15 +231,15 @@ ghip_status ghip_fit_forward(ghip_fit *f, const ghip_field_params *p, const floa
     const size_t bytes = f->field_words() * 4;
     // checkpoint 0 is the initial field; the others are written as reached
     HIP_TRY(hipMemcpyAsync(f->ckpt, u0, bytes, hipMemcpyDeviceToDevice, st));
-    float *cur = f->ckpt;
-    for (uint64_t m = 0; m < f->steps; ++m) {
-        float *dst;
-        if (m + 1 == f->steps) dst = u_final;
-        else if ((m + 1) % f->K == 0) dst = f->ckpt + ((m + 1) / f->K) * f->field_words();
-        else dst = f->lam[m & 1];  // the adjoint buffers are free during a forward pass
-        ghip_status s = forward_step(f, cur, dst, m, st);
+    const uint64_t n_seg = (f->steps + f->K - 1) / f->K;
+    for (uint64_t sg = 0; sg < n_seg; ++sg) {
+        const uint64_t m0 = sg * f->K;
+        const uint64_t len = std::min<uint64_t>(f->K, f->steps - m0);
+        float *last = m0 + len == f->steps ? u_final : f->ckpt + (sg + 1) * f->field_words();
+        // the segment's inner states go to the segment buffer, free until a backward pass
+        ghip_status s = forward_steps(f, f->ckpt + sg * f->field_words(), m0, len, f->seg + f->field_words(),
+                                      last, st);
         if (s != GHIP_OK) return s;
-        cur = dst;
     }
     f->have_forward = true;
     if (!stream) HIP_TRY(hipStreamSynchronize(st));
@@ -213,23 +252,30 @@ ghip_status ghip_fit_backward(ghip_fit *f, const float *grad_final, ghip_field_p
     if (!f->have_forward) return fail(GHIP_BAD_ARG, "backward needs a forward pass first");
     hipStream_t st = stream ? (hipStream_t)stream : f->own;
     const size_t fw = f->field_words();
-    HIP_TRY(hipMemsetAsync(f->part, 0, (size_t)f->S * fit_blocks(f->g) * kPartWords * sizeof(double), st));
+    HIP_TRY(hipMemsetAsync(f->part, 0, (size_t)f->S * f->part_blocks() * kPartWords * sizeof(double), st));
     HIP_TRY(hipMemcpyAsync(f->lam[0], grad_final, fw * 4, hipMemcpyDeviceToDevice, st));
     int li = 0;
     const uint64_t n_seg = (f->steps + f->K - 1) / f->K;
     for (uint64_t sg = n_seg; sg-- > 0;) {
         const uint64_t m0 = sg * f->K;
         const uint64_t len = std::min<uint64_t>(f->K, f->steps - m0);
-        // the segment's step starts
-        HIP_TRY(hipMemcpyAsync(f->seg, f->ckpt + sg * fw, fw * 4, hipMemcpyDeviceToDevice, st));
-        for (uint64_t j = 1; j < len; ++j) {
-            ghip_status s = forward_step(f, f->seg + (j - 1) * fw, f->seg + j * fw, m0 + j - 1, st);
+        // the segment's step starts: the checkpoint, then states 1 .. len - 1
+        const float *start0 = f->ckpt + sg * fw;
+        if (len > 1) {
+            ghip_status s = forward_steps(f, start0, m0, len - 1, f->seg + fw, f->seg + (len - 1) * fw, st);
             if (s != GHIP_OK) return s;
         }
         for (uint64_t j = len; j-- > 0;) {
             const uint64_t m = m0 + j;
+            if (f->fused) {
+                HIP_TRY(launch_fit_step_adjoint_fused(f->g, f->coef, f->S, f->max_sub, f->types + m * f->g.n,
+                                                      j == 0 ? start0 : f->seg + j * fw, f->lam[li],
+                                                      f->lam[li ^ 1], f->part, st));
+                li ^= 1;
+                continue;
+            }
             // substep states: state s - 1 feeds substep s; state 0 is the step start
-            const float *start = f->seg + j * fw;
+            const float *start = j == 0 ? start0 : f->seg + j * fw;
             for (int s = 1; s <= f->max_sub; ++s) {
                 const float *from = s == 1 ? start : f->sub + (size_t)(s - 2) * fw;
                 HIP_TRY(launch_fit_substep(f->g, f->coef, f->S, from, f->sub + (size_t)(s - 1) * fw, s, st));
@@ -246,7 +292,7 @@ ghip_status ghip_fit_backward(ghip_fit *f, const float *grad_final, ghip_field_p
         }
     }
     HIP_TRY(hipMemcpyAsync(grad_u0, f->lam[li], fw * 4, hipMemcpyDeviceToDevice, st));
-    HIP_TRY(launch_fit_reduce(f->g, f->S, f->part, f->grad, st));
+    HIP_TRY(launch_fit_reduce(f->S, f->part_blocks(), f->part, f->grad, st));
     std::vector<double> acc((size_t)f->S * kPartWords);
     HIP_TRY(hipMemcpyAsync(acc.data(), f->grad, acc.size() * sizeof(double), hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
