@@ -210,6 +210,7 @@ SweepArgs sweep_for(const ghip_sim *s, int cand, uint32_t *blocks) {
     const uint64_t words = (uint64_t)s->cap * s->cw;
     a.fold_per_block = cdiv(words, *blocks);
     a.xcd_chunk = 0;
+    a.reverse = 0;
     return a;
 }
 

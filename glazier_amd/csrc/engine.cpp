@@ -326,6 +326,7 @@ ghip_status run_sweep_pass(ghip_sim *s, int colour) {
     const uint64_t words = (uint64_t)s->cap * s->cw;
     a.fold_per_block = (uint32_t)((words + s->blocks - 1) / s->blocks);
     a.xcd_chunk = (s->blocks % 8 == 0) ? s->blocks / 8 : 0;
+    a.reverse = (int)(p & 1);
     TimedLaunch t{};
     HIP_TRY(s->begin_timed(0, &t));
     if (s->sweep_flags & kLength)

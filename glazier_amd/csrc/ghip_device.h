@@ -87,9 +87,27 @@ struct SweepArgs {
     uint32_t flags;
     uint32_t fold_per_block;
     int xcd_chunk;               // tiles per XCD in the remap, 0 for none
+    int reverse;                 // walk the tiles in reverse order (every other pass)
     int tiles_x, tiles_y, tiles_z;
     int off[26][3];
 };
+
+// Tile of block `blk` in a pass over `blocks` tiles. The hardware deals
+// workgroups to the eight XCDs in turn (block b to XCD b mod 8); with
+// `xcd_chunk` > 0 each XCD walks a contiguous run of tiles, so the halo rows
+// its tiles share stay in its L2. With `reverse` set the run is walked
+// backwards (the whole order when there is no run). The engine sets it on
+// every other pass: a pass then starts on the tiles the previous pass ended
+// with, whose lines are still in each XCD's L2. The draws do not depend on
+// the order, so neither does any result.
+__device__ __forceinline__ int pass_tile(int blk, int xcd_chunk, int reverse, int blocks) {
+    if (xcd_chunk > 0) {
+        int run = blk >> 3;
+        if (reverse) run = xcd_chunk - 1 - run;
+        return (blk & 7) * xcd_chunk + run;
+    }
+    return reverse ? blocks - 1 - blk : blk;
+}
 
 // Scalars of the plain-monolayer pass: the medium and one cell type.
 struct PlainParams {

@@ -85,10 +85,9 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &a, const int blk) {
     const int nthreads = blockDim.x;
     const DeviceTables *tab = a.tables;
 
-    // ---- block -> tile, keeping each XCD on a contiguous run of tiles so
-    // shared halo rows stay in one L2.
-    int b = blk;
-    if (a.xcd_chunk > 0) b = (b & 7) * a.xcd_chunk + (b >> 3);
+    // ---- block -> tile: each XCD on a contiguous run of tiles, walked
+    // backwards on every other pass
+    const int b = pass_tile(blk, a.xcd_chunk, a.reverse, a.tiles_x * a.tiles_y * (D3 ? a.tiles_z : 1));
     const int tile_x = b % a.tiles_x;
     const int tile_y = (b / a.tiles_x) % a.tiles_y;
     const int tile_z = D3 ? b / (a.tiles_x * a.tiles_y) : 0;

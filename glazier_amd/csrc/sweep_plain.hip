@@ -130,8 +130,7 @@ __global__ __launch_bounds__(NT) void sweep_plain2d_kernel(SweepArgs a, PlainPar
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
-    int b = blockIdx.x;
-    if (a.xcd_chunk > 0) b = (b & 7) * a.xcd_chunk + (b >> 3);
+    const int b = pass_tile(blockIdx.x, a.xcd_chunk, a.reverse, a.tiles_x * a.tiles_y);
     const int tile_x = b % a.tiles_x;
     const int tile_y = b / a.tiles_x;
     const int x0 = tile_x * PX, y0 = tile_y * PY;
@@ -222,25 +221,37 @@ __global__ __launch_bounds__(NT) void sweep_plain2d_kernel(SweepArgs a, PlainPar
         return tile[tl - kPitch - 1 + dxy];
     };
 
-    // ---- phase A: draw every target, queue the ones whose label would change
+    // ---- phase A: draw every target, queue the ones whose label would change.
+    // A lane's targets are q = k NT + tid; bit k of `moves` says target k
+    // would change. The wavefront reserves queue space once for all of them:
+    // a lane's offset is the prefix sum of the lanes' counts, taken from the
+    // ballots of the count's three bits.
+    constexpr int kPerLane = kTargets / NT;
+    static_assert(kPerLane <= 7, "a lane's count fits three bits");
+    uint32_t moves = 0;
 #pragma unroll
-    for (int k = 0; k < kTargets / NT; ++k) {
-        const int q = k * NT + tid;
+    for (int k = 0; k < kPerLane; ++k) {
         uint32_t site;
         int tl;
-        target(q, &site, &tl);
+        target(k * NT + tid, &site, &tl);
         const uint32_t h = site_draw(a.key, a.key2, site);
-        const uint32_t old = tile[tl];
-        const uint32_t nw = picked(h, tl);
-        const unsigned long long m = __ballot(old != nw);
-        if (m == 0) continue;
+        moves |= (tile[tl] != picked(h, tl) ? 1u : 0u) << k;
+    }
+    const uint32_t cnt = (uint32_t)__builtin_popcount(moves);
+    const unsigned long long c0 = __ballot(cnt & 1u), c1 = __ballot(cnt & 2u), c2 = __ballot(cnt & 4u);
+    const uint32_t total = (uint32_t)(__builtin_popcountll(c0) + 2 * __builtin_popcountll(c1) +
+                                      4 * __builtin_popcountll(c2));
+    if (total != 0) {
+        auto below = [](unsigned long long m) {
+            return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+        };
         uint32_t base = 0;
-        if (lane == 0) base = atomicAdd(&qcount, (uint32_t)__builtin_popcountll(m));
+        if (lane == 0) base = atomicAdd(&qcount, total);
         base = __builtin_amdgcn_readfirstlane(base);
-        if (old != nw) {
-            const uint32_t pos = base + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
-                                                                  __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-            queue[pos] = (uint16_t)q;
+        uint32_t pos = base + below(c0) + 2u * below(c1) + 4u * below(c2);
+#pragma unroll
+        for (int k = 0; k < kPerLane; ++k) {
+            if (moves & (1u << k)) queue[pos++] = (uint16_t)(k * NT + tid);
         }
     }
     __syncthreads();
