@@ -12,6 +12,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -30,7 +31,8 @@ thread_local std::string g_last_error;
 
 bool is_integer(double v) { return v == std::floor(v) && std::fabs(v) < 1048576.0; }
 
-const char *kFamilies[] = {"sweep", "fields", "moments", "population"};
+const char *kFamilies[] = {"sweep", "fields", "moments", "population", "step"};
+constexpr int kStepFamily = 4;
 
 }  // namespace
 
@@ -632,6 +634,12 @@ ghip_status ghip_step(ghip_sim *s, uint64_t steps) {
     if (s->traj && s->traj_len + steps > s->traj_cap)
         return fail(GHIP_BAD_ARG, "the recording has room for " + std::to_string(s->traj_cap - s->traj_len) +
                                       " more steps");
+    const uint64_t pass0 = s->pass;
+    if (s->span_timing) {
+        if (!s->span_start) HIP_TRY(hipEventCreate(&s->span_start));
+        if (!s->span_stop) HIP_TRY(hipEventCreate(&s->span_stop));
+        HIP_TRY(hipEventRecord(s->span_start, s->stream));
+    }
     if (s->resident && !s->traj) {
         ghip_status st = run_resident(s, steps);
         if (st != GHIP_OK) return st;
@@ -661,8 +669,15 @@ ghip_status ghip_step(ghip_sim *s, uint64_t steps) {
         s->mcs += 1;
         s->attempted += s->N;
     }
+    if (s->span_timing) HIP_TRY(hipEventRecord(s->span_stop, s->stream));
     HIP_TRY(hipStreamSynchronize(s->stream));
     if (s->timing) HIP_TRY(s->collect_timing());
+    if (s->span_timing) {
+        float elapsed = 0.0f;
+        HIP_TRY(hipEventElapsedTime(&elapsed, s->span_start, s->span_stop));
+        s->ms[kStepFamily] += elapsed;
+        s->launches[kStepFamily] += s->pass - pass0;
+    }
     if (s->has_pop) {
         uint32_t fl[4];
         HIP_TRY(hipMemcpy(fl, s->flags, 16, hipMemcpyDeviceToHost));
@@ -853,13 +868,15 @@ ghip_status ghip_cell_state(const ghip_sim *s, int64_t *out, size_t n, uint32_t 
 
 ghip_status ghip_set_timing(ghip_sim *s, int on) {
     if (!s) return fail(GHIP_BAD_ARG, "null handle");
-    s->timing = on != 0;
+    if (on < 0 || on > 2) return fail(GHIP_BAD_ARG, "timing mode is 0, 1 or 2");
+    s->timing = on == 1;
+    s->span_timing = on == 2;
     return GHIP_OK;
 }
 
 ghip_status ghip_kernel_time(const ghip_sim *s, const char *family, double *total_ms, uint64_t *launches) {
     if (!s || !family || !total_ms || !launches) return fail(GHIP_BAD_ARG, "null argument");
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < 5; ++i)
         if (std::strcmp(family, kFamilies[i]) == 0) {
             *total_ms = s->ms[i];
             *launches = s->launches[i];

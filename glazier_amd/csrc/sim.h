@@ -93,11 +93,13 @@ struct ghip_sim {
     bool resident = false;       // the whole lattice steps inside one workgroup
     int fx_log2 = 6, fy_log2 = 5, fz = 1, f_tiles_x = 1, f_tiles_y = 1, f_blocks = 1, f_kmax = 4;
 
-    bool timing = false;
+    bool timing = false;         // every launch bracketed by its own events
+    bool span_timing = false;    // one event pair around each ghip_step call
+    hipEvent_t span_start = nullptr, span_stop = nullptr;
     std::vector<ghip::TimedLaunch> pending;
     std::vector<ghip::TimedLaunch> spare;
-    double ms[4] = {0, 0, 0, 0};
-    uint64_t launches[4] = {0, 0, 0, 0};
+    double ms[5] = {0, 0, 0, 0, 0};       // sweep, fields, moments, population, step
+    uint64_t launches[5] = {0, 0, 0, 0, 0};
 
     ghip_batch *batch = nullptr;  // the batch this lattice is stepped in, if any
 
@@ -118,6 +120,8 @@ struct ghip_sim {
                 (void)hipEventDestroy(t.start);
                 (void)hipEventDestroy(t.stop);
             }
+        if (span_start) (void)hipEventDestroy(span_start);
+        if (span_stop) (void)hipEventDestroy(span_stop);
         if (stream) (void)hipStreamDestroy(stream);
     }
 

@@ -111,25 +111,129 @@ __device__ __forceinline__ PlainMove plain_move(const PlainParams &p, const uint
 }
 }  // namespace
 
+// ---- one tile's colour pass, shared by the streaming and the pipelined
+// kernels. The tile is a PX x PY block of sites at (x0, y0), landed in LDS
+// with one halo row above and below and one halo column either side: row r
+// of the lattice block (r = -1 .. PY) at word (r + 1) * (PX + 4) + 4.
+template <int PX, int PY>
+struct PlainTile {
+    static constexpr int kPitch = PX + 4;
+    static constexpr int kRows = PY + 2;
+    static constexpr int kWords = kRows * kPitch + 4;
+    static constexpr int kTargets = PX * PY / 4;
+    static constexpr int kTRow = PX / 2;  // targets per target row
+    static_assert(kTRow % 64 == 0, "a wavefront's targets share one row");
+    static_assert(kTargets <= 65536, "16-bit queue entries");
+
+    const uint32_t *tile;
+    int x0, y0;
+
+    // target q of the tile: row j (uniform in a wavefront), column i
+    __device__ __forceinline__ void target(const SweepArgs &a, int q, uint32_t *site, int *tl) const {
+        const int j = q / kTRow, i = q % kTRow;
+        *site = (uint32_t)(y0 + 2 * j + a.cy) * (uint32_t)a.w + (uint32_t)(x0 + a.cx + 2 * i);
+        *tl = (1 + 2 * j + a.cy) * kPitch + 4 + a.cx + 2 * i;
+    }
+    // pick k in ORDER3's planar order, (dx, dy) row by row: dx + 1 and dy + 1
+    // are the k-th 2-bit fields of 0x9224 and 0xA940
+    __device__ __forceinline__ uint32_t picked(uint32_t h, int tl) const {
+        const uint32_t sh = (h & 7u) * 2u;
+        const int dxy = (int)(((0xA940u >> sh) & 3u) * (uint32_t)kPitch + ((0x9224u >> sh) & 3u));
+        return tile[tl - kPitch - 1 + dxy];
+    }
+
+    // Phase A: draw every target, queue the ones whose label would change.
+    // A lane's targets are q = k NT + tid; bit k of `moves` says target k
+    // would change. The wavefront reserves queue space once for all of them:
+    // a lane's offset is the prefix sum of the lanes' counts, taken from the
+    // ballots of the count's three bits.
+    template <int NT>
+    __device__ __forceinline__ void phase_a(const SweepArgs &a, uint16_t *queue, uint32_t *qcount) const {
+        constexpr int kPerLane = kTargets / NT;
+        static_assert(kTargets % NT == 0 && kPerLane <= 7, "a lane's count fits three bits");
+        const int tid = threadIdx.x;
+        uint32_t moves = 0;
+#pragma unroll
+        for (int k = 0; k < kPerLane; ++k) {
+            uint32_t site;
+            int tl;
+            target(a, k * NT + tid, &site, &tl);
+            const uint32_t h = site_draw(a.key, a.key2, site);
+            moves |= (tile[tl] != picked(h, tl) ? 1u : 0u) << k;
+        }
+        const uint32_t cnt = (uint32_t)__builtin_popcount(moves);
+        const unsigned long long c0 = __ballot(cnt & 1u), c1 = __ballot(cnt & 2u), c2 = __ballot(cnt & 4u);
+        const uint32_t total = (uint32_t)(__builtin_popcountll(c0) + 2 * __builtin_popcountll(c1) +
+                                          4 * __builtin_popcountll(c2));
+        if (total == 0) return;
+        auto below = [](unsigned long long m) {
+            return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+        };
+        uint32_t base = 0;
+        if ((tid & 63) == 0) base = atomicAdd(qcount, total);
+        base = __builtin_amdgcn_readfirstlane(base);
+        uint32_t pos = base + below(c0) + 2u * below(c1) + 4u * below(c2);
+#pragma unroll
+        for (int k = 0; k < kPerLane; ++k) {
+            if (moves & (1u << k)) queue[pos++] = (uint16_t)(k * NT + tid);
+        }
+    }
+
+    // Phase B: price the nq queued targets; returns this lane's accepted
+    // copies. It redraws each target's hash (cheaper than holding it in LDS)
+    // and issues the four per-label gathers together.
+    template <int NT>
+    __device__ __forceinline__ uint32_t phase_b(const SweepArgs &a, const PlainParams &p, const uint32_t *thr_s,
+                                                const uint16_t *queue, uint32_t nq, bool has_medium) const {
+        const uint32_t cw = (uint32_t)a.cell_words;
+        uint32_t my_acc = 0;
+        for (uint32_t qi = threadIdx.x; qi < nq; qi += NT) {
+            uint32_t site;
+            int tl;
+            target(a, queue[qi], &site, &tl);
+            const uint32_t h = site_draw(a.key, a.key2, site);
+            const uint32_t r24 = h >> 8;
+            const uint32_t old = tile[tl];
+            const uint32_t nw = picked(h, tl);
+            const uint32_t oid = label_id(old), nid = label_id(nw);
+            // the four gathers go out together, before anything waits on them;
+            // label 0's record is real memory whose value is never used
+            const int64_t ao = a.a_cur[oid * cw], eo = a.e_prev[oid * cw];
+            const int64_t an = a.a_cur[nid * cw], en = a.e_prev[nid * cw];
+            const uint32_t n0 = tile[tl - kPitch - 1], n1 = tile[tl - kPitch], n2 = tile[tl - kPitch + 1];
+            const uint32_t n3 = tile[tl - 1], n4 = tile[tl + 1];
+            const uint32_t n5 = tile[tl + kPitch - 1], n6 = tile[tl + kPitch], n7 = tile[tl + kPitch + 1];
+            const PlainMove mv =
+                plain_move(p, thr_s, r24, old, nw, n0, n1, n2, n3, n4, n5, n6, n7, ao + eo, an + en, has_medium);
+            if (!mv.acc) continue;
+            a.labels[site] = nw;
+            if (oid != 0)
+                atomicAdd(reinterpret_cast<unsigned long long *>(a.f_acc + oid * cw),
+                          (unsigned long long)pack_vs(-1, mv.ds_old));
+            if (nid != 0)
+                atomicAdd(reinterpret_cast<unsigned long long *>(a.f_acc + nid * cw),
+                          (unsigned long long)pack_vs(1, mv.ds_new));
+            ++my_acc;
+        }
+        return my_acc;
+    }
+};
+
 template <int PX, int PY, int NT>
 __global__ __launch_bounds__(NT) void sweep_plain2d_kernel(SweepArgs a, PlainParams p) {
-    constexpr int kPitch = PX + 4;
-    constexpr int kRows = PY + 2;
-    constexpr int kTargets = PX * PY / 4;
+    using T = PlainTile<PX, PY>;
+    constexpr int kPitch = T::kPitch;
+    constexpr int kRows = T::kRows;
     constexpr int kC4 = PX / 4;                          // 16-byte loads per row
     constexpr int kLoads = (kRows * kC4 + NT - 1) / NT;  // per lane
-    constexpr int kTRow = PX / 2;                        // targets per target row
-    static_assert(kTRow % 64 == 0, "a wavefront's targets share one row");
-    static_assert(kTargets % NT == 0 && kTargets <= 65536, "targets per lane, 16-bit queue");
 
-    __shared__ __attribute__((aligned(16))) uint32_t tile[kRows * kPitch + 4];
-    __shared__ uint16_t queue[kTargets];
+    __shared__ __attribute__((aligned(16))) uint32_t tile[T::kWords];
+    __shared__ uint16_t queue[T::kTargets];
     __shared__ uint32_t thr_s[kPThr];
     __shared__ uint32_t qcount;
     __shared__ uint32_t acc_block;
 
     const int tid = threadIdx.x;
-    const int lane = tid & 63;
     const int b = pass_tile(blockIdx.x, a.xcd_chunk, a.reverse, a.tiles_x * a.tiles_y);
     const int tile_x = b % a.tiles_x;
     const int tile_y = b / a.tiles_x;
@@ -207,89 +311,10 @@ __global__ __launch_bounds__(NT) void sweep_plain2d_kernel(SweepArgs a, PlainPar
     }
     const bool has_medium = __syncthreads_or(p.needs_medium && lo == 0u) != 0;
 
-    // target q of the tile: row j (uniform in a wavefront), column i
-    auto target = [&](int q, uint32_t *site, int *tl) {
-        const int j = q / kTRow, i = q % kTRow;
-        *site = (uint32_t)(y0 + 2 * j + a.cy) * (uint32_t)W + (uint32_t)(x0 + a.cx + 2 * i);
-        *tl = (1 + 2 * j + a.cy) * kPitch + 4 + a.cx + 2 * i;
-    };
-    // pick k in ORDER3's planar order, (dx, dy) row by row: dx + 1 and dy + 1
-    // are the k-th 2-bit fields of 0x9224 and 0xA940
-    auto picked = [&](uint32_t h, int tl) {
-        const uint32_t sh = (h & 7u) * 2u;
-        const int dxy = (int)(((0xA940u >> sh) & 3u) * (uint32_t)kPitch + ((0x9224u >> sh) & 3u));
-        return tile[tl - kPitch - 1 + dxy];
-    };
-
-    // ---- phase A: draw every target, queue the ones whose label would change.
-    // A lane's targets are q = k NT + tid; bit k of `moves` says target k
-    // would change. The wavefront reserves queue space once for all of them:
-    // a lane's offset is the prefix sum of the lanes' counts, taken from the
-    // ballots of the count's three bits.
-    constexpr int kPerLane = kTargets / NT;
-    static_assert(kPerLane <= 7, "a lane's count fits three bits");
-    uint32_t moves = 0;
-#pragma unroll
-    for (int k = 0; k < kPerLane; ++k) {
-        uint32_t site;
-        int tl;
-        target(k * NT + tid, &site, &tl);
-        const uint32_t h = site_draw(a.key, a.key2, site);
-        moves |= (tile[tl] != picked(h, tl) ? 1u : 0u) << k;
-    }
-    const uint32_t cnt = (uint32_t)__builtin_popcount(moves);
-    const unsigned long long c0 = __ballot(cnt & 1u), c1 = __ballot(cnt & 2u), c2 = __ballot(cnt & 4u);
-    const uint32_t total = (uint32_t)(__builtin_popcountll(c0) + 2 * __builtin_popcountll(c1) +
-                                      4 * __builtin_popcountll(c2));
-    if (total != 0) {
-        auto below = [](unsigned long long m) {
-            return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-        };
-        uint32_t base = 0;
-        if (lane == 0) base = atomicAdd(&qcount, total);
-        base = __builtin_amdgcn_readfirstlane(base);
-        uint32_t pos = base + below(c0) + 2u * below(c1) + 4u * below(c2);
-#pragma unroll
-        for (int k = 0; k < kPerLane; ++k) {
-            if (moves & (1u << k)) queue[pos++] = (uint16_t)(k * NT + tid);
-        }
-    }
+    const PlainTile<PX, PY> pt{tile, x0, y0};
+    pt.template phase_a<NT>(a, queue, &qcount);
     __syncthreads();
-
-    // ---- phase B: price the queued targets
-    const uint32_t nq = qcount;
-    const uint32_t cw = (uint32_t)a.cell_words;
-    uint32_t my_acc = 0;
-    for (uint32_t qi = tid; qi < nq; qi += NT) {
-        uint32_t site;
-        int tl;
-        target(queue[qi], &site, &tl);
-        const uint32_t h = site_draw(a.key, a.key2, site);
-        const uint32_t r24 = h >> 8;
-        const uint32_t old = tile[tl];
-        const uint32_t nw = picked(h, tl);
-        const uint32_t oid = label_id(old), nid = label_id(nw);
-        // the four gathers go out together, before anything waits on them;
-        // label 0's record is real memory whose value is never used
-        const int64_t ao = a.a_cur[oid * cw], eo = a.e_prev[oid * cw];
-        const int64_t an = a.a_cur[nid * cw], en = a.e_prev[nid * cw];
-        const uint32_t n0 = tile[tl - kPitch - 1], n1 = tile[tl - kPitch], n2 = tile[tl - kPitch + 1];
-        const uint32_t n3 = tile[tl - 1], n4 = tile[tl + 1];
-        const uint32_t n5 = tile[tl + kPitch - 1], n6 = tile[tl + kPitch], n7 = tile[tl + kPitch + 1];
-        const PlainMove mv =
-            plain_move(p, thr_s, r24, old, nw, n0, n1, n2, n3, n4, n5, n6, n7, ao + eo, an + en, has_medium);
-        const bool acc = mv.acc;
-        const int ds_old = mv.ds_old, ds_new = mv.ds_new;
-        if (!acc) continue;
-        a.labels[site] = nw;
-        if (oid != 0)
-            atomicAdd(reinterpret_cast<unsigned long long *>(a.f_acc + oid * cw),
-                      (unsigned long long)pack_vs(-1, ds_old));
-        if (nid != 0)
-            atomicAdd(reinterpret_cast<unsigned long long *>(a.f_acc + nid * cw),
-                      (unsigned long long)pack_vs(1, ds_new));
-        ++my_acc;
-    }
+    const uint32_t my_acc = pt.template phase_b<NT>(a, p, thr_s, queue, qcount, has_medium);
     if (my_acc) atomicAdd(&acc_block, my_acc);
     __syncthreads();
     if (tid == 0 && acc_block)

@@ -233,8 +233,11 @@ class HipSimulation:
         return out.reshape(n, words.value)
 
     # -- timing -----------------------------------------------------------
-    def set_timing(self, on: bool) -> None:
-        _check(library().ghip_set_timing(self._h, 1 if on else 0))
+    def set_timing(self, on) -> None:
+        """True (or 1): events around every launch; 2: one event pair around
+        each step call, summed under family "step"; False/0: off."""
+        mode = int(on) if not isinstance(on, bool) else (1 if on else 0)
+        _check(library().ghip_set_timing(self._h, mode))
 
     def kernel_time(self, family: str):
         ms, n = ctypes.c_double(), ctypes.c_uint64()

@@ -181,10 +181,15 @@ ghip_status ghip_counters(const ghip_sim *sim, uint64_t *attempted, uint64_t *ac
 ghip_status ghip_cell_state(const ghip_sim *sim, int64_t *out, size_t n,
                             uint32_t *words_per_label);
 
-/* Kernel timing over the stream the engine launches on: when on, every sweep
- * pass, field step and moment rebuild is bracketed by HIP events.
+/* Kernel timing over the stream the engine launches on. Mode 1: every sweep
+ * pass, field step and moment rebuild is bracketed by HIP events. Mode 2:
+ * one event pair brackets each ghip_step call, and family "step" adds up
+ * those spans, its launch count being the colour passes they ran (so on a
+ * model whose step is its colour passes alone, milliseconds per launch is
+ * the mean pass with the dispatch gaps between passes, and no events
+ * between them). 0 turns timing off; other modes are GHIP_BAD_ARG.
  * ghip_kernel_time returns the summed milliseconds and the launch count of
- * one kernel family: "sweep", "fields", "moments" or "population". */
+ * one family: "sweep", "fields", "moments", "population" or "step". */
 ghip_status ghip_set_timing(ghip_sim *sim, int on);
 ghip_status ghip_kernel_time(const ghip_sim *sim, const char *family, double *total_ms,
                              uint64_t *launches);

@@ -95,3 +95,21 @@ def test_a_resident_lattice_matches_across_launch_and_call_boundaries(engine, or
     assert gpu.counters() == emu.counters()
     assert gpu.mcs == steps
     gpu.close()
+
+
+def test_span_timing_counts_the_passes_of_each_step_call(engine):
+    from glazier_amd import HipSimulation
+    from glazier_amd.engine import GhipError
+
+    m, labels, types, _, _, _ = case("surface1024")
+    gpu = HipSimulation(m, labels, types)
+    gpu.set_timing(2)
+    gpu.step(3)
+    gpu.step(2)
+    gpu.set_timing(False)
+    ms, passes = gpu.kernel_time("step")
+    assert passes == 4 * 5 and ms > 0.0
+    assert gpu.kernel_time("sweep") == (0.0, 0)
+    with pytest.raises(GhipError):
+        gpu.set_timing(3)
+    gpu.close()
