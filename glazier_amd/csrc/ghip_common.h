@@ -39,11 +39,15 @@ GHD uint32_t pass_key(uint64_t seed, uint32_t replica, uint64_t step, uint32_t c
 
 GHD uint32_t second_key(uint32_t key) { return hash32(key ^ 0x85EBCA6BU); }
 
-// The one draw a target takes in a pass: a bijection of the site for a fixed
-// pass, so no two targets of a pass share a draw.
-GHD uint32_t site_draw(uint32_t key, uint32_t key2, uint32_t site) {
-    return hash32(hash32(site ^ key) ^ key2);
-}
+// The draws of one target in a pass, two rounds of the hash. The first,
+// p = site_pick(key, site), is a bijection of the site for a fixed pass (no
+// two targets of a pass share it); its low bits pick the neighbour when the
+// neighbour count is a power of two. The second, h = site_draw(key2, p),
+// holds the acceptance uniform in its top 24 bits (and the pick for other
+// neighbour counts). Asking only which neighbour a target picks takes one
+// round.
+GHD uint32_t site_pick(uint32_t key, uint32_t site) { return hash32(site ^ key); }
+GHD uint32_t site_draw(uint32_t key2, uint32_t p) { return hash32(p ^ key2); }
 
 // Second 24-bit uniform, used when the neighbour count is not a power of two
 // and the pick has consumed the whole first draw.

@@ -164,12 +164,13 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &a, const int blk) {
         const int lx = halo + 2 * i + a.cx, ly = halo + 2 * j + a.cy, lz = D3 ? halo + 2 * k + a.cz : 0;
         const int tl = (lz * LY + ly) * LX + lx;
         const uint32_t site = (uint32_t)(((size_t)gz * a.h + gy) * (size_t)a.w + gx);
-        const uint32_t h = site_draw(a.key, a.key2, site);
+        const uint32_t ph = site_pick(a.key, site);
+        const uint32_t h = site_draw(a.key2, ph);
         int pick;
         uint32_t r24;
         int pdx, pdy, pdz;
         if (MOORE2D) {
-            pick = (int)(h & 7u);
+            pick = (int)(ph & 7u);
             r24 = h >> 8;
             int q = pick + (pick >= 4 ? 1 : 0);
             pdy = q / 3 - 1;
@@ -178,7 +179,7 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &a, const int blk) {
         } else {
             const int n = a.n_off;
             if ((n & (n - 1)) == 0) {
-                pick = (int)(h & (uint32_t)(n - 1));
+                pick = (int)(ph & (uint32_t)(n - 1));
                 r24 = h >> 8;
             } else {
                 pick = (int)(((uint64_t)h * (uint64_t)n) >> 32);

@@ -134,10 +134,11 @@ struct PlainTile {
         *site = (uint32_t)(y0 + 2 * j + a.cy) * (uint32_t)a.w + (uint32_t)(x0 + a.cx + 2 * i);
         *tl = (1 + 2 * j + a.cy) * kPitch + 4 + a.cx + 2 * i;
     }
-    // pick k in ORDER3's planar order, (dx, dy) row by row: dx + 1 and dy + 1
-    // are the k-th 2-bit fields of 0x9224 and 0xA940
-    __device__ __forceinline__ uint32_t picked(uint32_t h, int tl) const {
-        const uint32_t sh = (h & 7u) * 2u;
+    // the neighbour a target's pick round `ph` names: k = ph & 7 in ORDER3's
+    // planar order, (dx, dy) row by row; dx + 1 and dy + 1 are the k-th 2-bit
+    // fields of 0x9224 and 0xA940
+    __device__ __forceinline__ uint32_t picked(uint32_t ph, int tl) const {
+        const uint32_t sh = (ph & 7u) * 2u;
         const int dxy = (int)(((0xA940u >> sh) & 3u) * (uint32_t)kPitch + ((0x9224u >> sh) & 3u));
         return tile[tl - kPitch - 1 + dxy];
     }
@@ -158,8 +159,7 @@ struct PlainTile {
             uint32_t site;
             int tl;
             target(a, k * NT + tid, &site, &tl);
-            const uint32_t h = site_draw(a.key, a.key2, site);
-            moves |= (tile[tl] != picked(h, tl) ? 1u : 0u) << k;
+            moves |= (tile[tl] != picked(site_pick(a.key, site), tl) ? 1u : 0u) << k;
         }
         const uint32_t cnt = (uint32_t)__builtin_popcount(moves);
         const unsigned long long c0 = __ballot(cnt & 1u), c1 = __ballot(cnt & 2u), c2 = __ballot(cnt & 4u);
@@ -180,8 +180,9 @@ struct PlainTile {
     }
 
     // Phase B: price the nq queued targets; returns this lane's accepted
-    // copies. It redraws each target's hash (cheaper than holding it in LDS)
-    // and issues the four per-label gathers together.
+    // copies. It redraws each target's pick round (cheaper than holding it in
+    // LDS), takes the uniform's round, and issues the four per-label gathers
+    // together.
     template <int NT>
     __device__ __forceinline__ uint32_t phase_b(const SweepArgs &a, const PlainParams &p, const uint32_t *thr_s,
                                                 const uint16_t *queue, uint32_t nq, bool has_medium) const {
@@ -191,10 +192,10 @@ struct PlainTile {
             uint32_t site;
             int tl;
             target(a, queue[qi], &site, &tl);
-            const uint32_t h = site_draw(a.key, a.key2, site);
-            const uint32_t r24 = h >> 8;
+            const uint32_t ph = site_pick(a.key, site);
+            const uint32_t r24 = site_draw(a.key2, ph) >> 8;
             const uint32_t old = tile[tl];
-            const uint32_t nw = picked(h, tl);
+            const uint32_t nw = picked(ph, tl);
             const uint32_t oid = label_id(old), nid = label_id(nw);
             // the four gathers go out together, before anything waits on them;
             // label 0's record is real memory whose value is never used
@@ -385,8 +386,7 @@ __global__ __launch_bounds__(kResThreads) void sweep_plain2d_resident_kernel(Pla
                     const int x = 2 * (int)(q & ((1u << tw) - 1u)) + cx;
                     const int y = 2 * (int)(q >> tw) + cy;
                     site = ((uint32_t)y << wl) | (uint32_t)x;
-                    const uint32_t h = site_draw(key, key2, site);
-                    const uint32_t sh = (h & 7u) * 2u;
+                    const uint32_t sh = (site_pick(key, site) & 7u) * 2u;
                     const int nx = (x + (int)((0x9224u >> sh) & 3u) - 1) & (W - 1);
                     int ny = y + (int)((0xA940u >> sh) & 3u) - 1;
                     ny += ny < 0 ? H : 0;
@@ -411,7 +411,7 @@ __global__ __launch_bounds__(kResThreads) void sweep_plain2d_resident_kernel(Pla
             for (uint32_t qi = tid; qi < nq; qi += kResThreads) {
                 const uint32_t site = queue[qi];
                 const int x = (int)(site & (uint32_t)(W - 1)), y = (int)(site >> wl);
-                const uint32_t h = site_draw(key, key2, site);
+                const uint32_t ph = site_pick(key, site);
                 const uint32_t xm = (uint32_t)((x - 1) & (W - 1)), xp = (uint32_t)((x + 1) & (W - 1));
                 const uint32_t ym = (uint32_t)(y == 0 ? H - 1 : y - 1) << wl;
                 const uint32_t y0 = (uint32_t)y << wl;
@@ -421,11 +421,12 @@ __global__ __launch_bounds__(kResThreads) void sweep_plain2d_resident_kernel(Pla
                 const uint32_t n5 = lat[yp | xm], n6 = lat[yp | (uint32_t)x], n7 = lat[yp | xp];
                 const uint32_t old = lat[site];
                 // the picked neighbour, k in ORDER3's planar order
-                const uint32_t k = h & 7u;
+                const uint32_t k = ph & 7u;
                 const uint32_t nw = k == 0 ? n0 : k == 1 ? n1 : k == 2 ? n2 : k == 3 ? n3
                                   : k == 4 ? n4 : k == 5 ? n5 : k == 6 ? n6 : n7;
                 const uint32_t oid = label_id(old), nid = label_id(nw);
-                const PlainMove mv = plain_move(p, thr_s, h >> 8, old, nw, n0, n1, n2, n3, n4, n5, n6, n7,
+                const uint32_t r24 = site_draw(key2, ph) >> 8;
+                const PlainMove mv = plain_move(p, thr_s, r24, old, nw, n0, n1, n2, n3, n4, n5, n6, n7,
                                                 rec[oid], rec[nid], p.needs_medium != 0);
                 if (!mv.acc) continue;
                 lat[site] = nw;

@@ -97,6 +97,7 @@ def library() -> ctypes.CDLL:
         "oracle_principal_axis": ([P(f64), P(f64), P(f64)], None),
         "oracle_hash32": ([u32], u32),
         "oracle_pass_key": ([u64, u32, u64, u32], u32),
+        "oracle_site_pick": ([u32, u32], u32),
         "oracle_site_draw": ([u32, u32], u32),
         "emu_create": ([P(CModel), P(u32), P(ctypes.c_uint8), sz, P(f64), P(f64)], v),
         "emu_destroy": ([v], None),

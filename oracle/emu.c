@@ -19,10 +19,11 @@
  *     then fields (diffusion substeps, then secretion/uptake), activity
  *     decay, and division/death.
  *
- *   Each target draws one 32-bit counter hash h of (pass key, site): the
- *   neighbour pick comes from its low bits and the acceptance uniform from
- *   its top 24 bits (a second hash when the neighbour count is not a power
- *   of two).
+ *   Each target takes two rounds of a 32-bit counter hash: p of (pass key,
+ *   site), whose low bits pick the neighbour when the neighbour count is a
+ *   power of two, and h of (second key, p), whose top 24 bits are the
+ *   acceptance uniform (for other neighbour counts h also gives the pick,
+ *   and a third round the uniform).
  *
  * Energies are f32 in a fixed order (below); the length term and the
  * division geometry are f64 from exact integer moments. Compiled with
@@ -63,9 +64,11 @@ uint32_t oracle_pass_key(uint64_t seed, uint32_t replica, uint64_t step, uint32_
 
 static uint32_t second_key(uint32_t key) { return oracle_hash32(key ^ 0x85EBCA6BU); }
 
-/* The draw of one site in one pass. */
+/* The two rounds of one site's draw in one pass: the pick round, and the
+ * round after it whose top 24 bits are the acceptance uniform. */
+uint32_t oracle_site_pick(uint32_t key, uint32_t site) { return oracle_hash32(site ^ key); }
 uint32_t oracle_site_draw(uint32_t key, uint32_t site) {
-    return oracle_hash32(oracle_hash32(site ^ key) ^ second_key(key));
+    return oracle_hash32(oracle_site_pick(key, site) ^ second_key(key));
 }
 
 /* Colour order of one step: a Fisher-Yates shuffle driven by one hash chain. */
@@ -831,11 +834,12 @@ static void one_pass(emu_sim *e, int colour) {
         for (int64_t y = cy; y < e->h; y += 2) {
             for (int64_t x = cx; x < e->w; x += 2) {
                 size_t t = e_index(e, x, y, z);
-                uint32_t h = oracle_hash32(oracle_hash32((uint32_t)t ^ key) ^ key2);
+                uint32_t ph = oracle_hash32((uint32_t)t ^ key);
+                uint32_t h = oracle_hash32(ph ^ key2);
                 int pick;
                 uint32_t r24;
                 if (pow2) {
-                    pick = (int)(h & (uint32_t)(e->n_off - 1));
+                    pick = (int)(ph & (uint32_t)(e->n_off - 1));
                     r24 = h >> 8;
                 } else {
                     pick = (int)(((uint64_t)h * (uint64_t)e->n_off) >> 32);

@@ -86,6 +86,7 @@ typedef struct emu_sim emu_sim;
 
 uint32_t oracle_hash32(uint32_t x);
 uint32_t oracle_pass_key(uint64_t seed, uint32_t replica, uint64_t step, uint32_t colour);
+uint32_t oracle_site_pick(uint32_t key, uint32_t site);
 uint32_t oracle_site_draw(uint32_t key, uint32_t site);
 
 emu_sim *emu_create(const ghip_model *m, const uint32_t *labels, const uint8_t *cell_type,

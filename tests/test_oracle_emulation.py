@@ -220,16 +220,23 @@ def test_counter_hash_is_a_bijection_and_uniform(oracle_lib):
     lib = oracle.library()
     key = lib.oracle_pass_key(7, 0, 3, 1)
     draws = np.array([lib.oracle_site_draw(key, s) for s in range(65536)], dtype=np.uint64)
-    picks = np.bincount((draws & 7).astype(np.int64), minlength=8)
+    pick_rounds = np.array([lib.oracle_site_pick(key, s) for s in range(65536)], dtype=np.uint64)
+    picks = np.bincount((pick_rounds & 7).astype(np.int64), minlength=8)
     chi2 = ((picks - 8192.0) ** 2 / 8192.0).sum()
     assert chi2 < 30.0  # 7 dof, p ~ 1e-4
     top = np.bincount((draws >> 24).astype(np.int64), minlength=256)
     chi2 = ((top - 256.0) ** 2 / 256.0).sum()
     assert chi2 < 340.0  # 255 dof, p ~ 3e-4
-    # neighbouring sites' draws are uncorrelated
+    # neighbouring sites' draws are uncorrelated, and a site's pick is
+    # independent of its uniform
     u = (draws >> 8).astype(np.float64) / 2**24
     r = np.corrcoef(u[:-2], u[2:])[0, 1]
     assert abs(r) < 0.02
+    k = (pick_rounds & 7).astype(np.int64)
+    means = np.array([u[k == j].mean() for j in range(8)])
+    assert np.abs(means - 0.5).max() < 0.015  # ~8192 draws per pick: sd 0.0032
+    assert np.array_equal(draws, [oracle.hash32(int(p) ^ oracle.hash32(key ^ 0x85EBCA6B))
+                                  for p in pick_rounds])
 
 
 def test_f32_fields_track_the_f64_solver(oracle_lib):

@@ -161,7 +161,7 @@ def test_emulated_nematic_changes_match_a_from_scratch_computation(oracle_lib):
     rng = np.random.default_rng(3)
     w = m.width
     checked = 0
-    for _ in range(300):
+    for _ in range(600):
         t = int(rng.integers(lab.size))
         x, y = t % w, t // w
         dx, dy = rng.integers(-1, 2, size=2)
