@@ -179,15 +179,24 @@ struct PlainTile {
         }
     }
 
-    // Phase B: price the nq queued targets; returns this lane's accepted
-    // copies. It redraws each target's pick round (cheaper than holding it in
-    // LDS), takes the uniform's round, and issues the four per-label gathers
-    // together.
+    // Phase B: price the nq queued targets, write the accepted copies'
+    // labels, and return this lane's accepted count. It redraws each
+    // target's pick round (cheaper than holding it in LDS), takes the
+    // uniform's round, and issues the four per-label gathers together. The
+    // record changes of a lane's first entry (queue entry tid) leave in
+    // d_old and d_new as id | (ds + 8) / 2 << 28 (the surface change ds is
+    // even, -8..8; the volume changes by -1 and +1), or kEmpty, for
+    // merge_deltas; those of later entries (a tile with more than NT queued
+    // targets) go straight to the records.
+    static constexpr uint32_t kEmpty = 0xFFFFFFFFu;
+
     template <int NT>
     __device__ __forceinline__ uint32_t phase_b(const SweepArgs &a, const PlainParams &p, const uint32_t *thr_s,
-                                                const uint16_t *queue, uint32_t nq, bool has_medium) const {
+                                                const uint16_t *queue, uint32_t nq, bool has_medium,
+                                                uint32_t &d_old, uint32_t &d_new) const {
         const uint32_t cw = (uint32_t)a.cell_words;
         uint32_t my_acc = 0;
+        d_old = d_new = kEmpty;
         for (uint32_t qi = threadIdx.x; qi < nq; qi += NT) {
             uint32_t site;
             int tl;
@@ -208,15 +217,68 @@ struct PlainTile {
                 plain_move(p, thr_s, r24, old, nw, n0, n1, n2, n3, n4, n5, n6, n7, ao + eo, an + en, has_medium);
             if (!mv.acc) continue;
             a.labels[site] = nw;
-            if (oid != 0)
-                atomicAdd(reinterpret_cast<unsigned long long *>(a.f_acc + oid * cw),
-                          (unsigned long long)pack_vs(-1, mv.ds_old));
-            if (nid != 0)
-                atomicAdd(reinterpret_cast<unsigned long long *>(a.f_acc + nid * cw),
-                          (unsigned long long)pack_vs(1, mv.ds_new));
+            if (qi < (uint32_t)NT) {
+                if (oid != 0) d_old = oid | (uint32_t)((mv.ds_old + 8) >> 1) << kTypeShift;
+                if (nid != 0) d_new = nid | (uint32_t)((mv.ds_new + 8) >> 1) << kTypeShift;
+            } else {
+                if (oid != 0)
+                    atomicAdd(reinterpret_cast<unsigned long long *>(a.f_acc + oid * cw),
+                              (unsigned long long)pack_vs(-1, mv.ds_old));
+                if (nid != 0)
+                    atomicAdd(reinterpret_cast<unsigned long long *>(a.f_acc + nid * cw),
+                              (unsigned long long)pack_vs(1, mv.ds_new));
+            }
             ++my_acc;
         }
         return my_acc;
+    }
+
+    // The record changes of the tile's accepted copies, merged per label
+    // before they leave the workgroup: once every queue entry has been read,
+    // the queue's LDS holds a small open-addressed table of (label, int64
+    // delta); the copies add into it with LDS atomics, and then each label
+    // the tile touched costs one global atomic instead of one per copy (a
+    // label that finds no slot in a few probes goes straight to its record).
+    // Integer sums in any order: the records are exactly those of one atomic
+    // per copy. Called by the whole workgroup.
+    static constexpr int kSlots = kTargets >= 2048 ? 256 : kTargets >= 1024 ? 64 : 32;
+    static_assert(kSlots * 12 <= kTargets * 2, "the table fits in the queue's bytes");
+
+    template <int NT>
+    __device__ __forceinline__ void merge_deltas(const SweepArgs &a, uint16_t *queue, uint32_t d_old,
+                                                 uint32_t d_new) const {
+        const int tid = threadIdx.x;
+        const uint32_t cw = (uint32_t)a.cell_words;
+        unsigned long long *vals = reinterpret_cast<unsigned long long *>(queue);
+        uint32_t *keys = reinterpret_cast<uint32_t *>(vals + kSlots);
+        for (int i = tid; i < kSlots; i += NT) {
+            keys[i] = kEmpty;
+            vals[i] = 0ull;
+        }
+        __syncthreads();
+        auto add = [&](uint32_t packed, int64_t dv) {
+            const uint32_t id = label_id(packed);
+            const int64_t d = pack_vs(dv, 2 * (int64_t)(packed >> kTypeShift) - 8);
+            uint32_t h = (id * 0x9E3779B1u) >> 24;
+#pragma unroll
+            for (int probe = 0; probe < 4; ++probe) {
+                h &= (uint32_t)(kSlots - 1);
+                const uint32_t k = atomicCAS(&keys[h], kEmpty, id);
+                if (k == kEmpty || k == id) {
+                    atomicAdd(&vals[h], (unsigned long long)d);
+                    return;
+                }
+                ++h;
+            }
+            atomicAdd(reinterpret_cast<unsigned long long *>(a.f_acc + id * cw), (unsigned long long)d);
+        };
+        if (d_old != kEmpty) add(d_old, -1);
+        if (d_new != kEmpty) add(d_new, 1);
+        __syncthreads();
+        for (int i = tid; i < kSlots; i += NT) {
+            const uint32_t id = keys[i];
+            if (id != kEmpty) atomicAdd(reinterpret_cast<unsigned long long *>(a.f_acc + id * cw), vals[i]);
+        }
     }
 };
 
@@ -229,7 +291,7 @@ __global__ __launch_bounds__(NT) void sweep_plain2d_kernel(SweepArgs a, PlainPar
     constexpr int kLoads = (kRows * kC4 + NT - 1) / NT;  // per lane
 
     __shared__ __attribute__((aligned(16))) uint32_t tile[T::kWords];
-    __shared__ uint16_t queue[T::kTargets];
+    __shared__ __attribute__((aligned(16))) uint16_t queue[T::kTargets];  // then the delta table
     __shared__ uint32_t thr_s[kPThr];
     __shared__ uint32_t qcount;
     __shared__ uint32_t acc_block;
@@ -315,11 +377,15 @@ __global__ __launch_bounds__(NT) void sweep_plain2d_kernel(SweepArgs a, PlainPar
     const PlainTile<PX, PY> pt{tile, x0, y0};
     pt.template phase_a<NT>(a, queue, &qcount);
     __syncthreads();
-    const uint32_t my_acc = pt.template phase_b<NT>(a, p, thr_s, queue, qcount, has_medium);
+    uint32_t d_old, d_new;
+    const uint32_t my_acc = pt.template phase_b<NT>(a, p, thr_s, queue, qcount, has_medium, d_old, d_new);
     if (my_acc) atomicAdd(&acc_block, my_acc);
     __syncthreads();
-    if (tid == 0 && acc_block)
-        atomicAdd(a.accepted + (blockIdx.x & (kAcceptSlots - 1)), (unsigned long long)acc_block);
+    // every queue entry is read; a tile with accepted copies merges their
+    // record changes (a frozen tile takes no further barrier)
+    const uint32_t acc = acc_block;
+    if (acc != 0) pt.template merge_deltas<NT>(a, queue, d_old, d_new);
+    if (tid == 0 && acc != 0) atomicAdd(a.accepted + (blockIdx.x & (kAcceptSlots - 1)), (unsigned long long)acc);
 }
 
 // ---- the resident pass: a small lattice stepped whole inside one workgroup

@@ -53,6 +53,14 @@ def case(name):
     if name == "surface1024":
         m, l, t = plain_monolayer(1024, 3, True)
         return m, l, t, None, None, 3
+    if name == "scattered1024":
+        # the plain fast path on 1024^2 with every cell's 64 sites scattered
+        # at random: nearly every target is queued (more than one per lane),
+        # and a tile's accepted copies touch more labels than its merge table
+        # holds
+        m, l, t = plain_monolayer(1024, 4, False)
+        l = np.random.default_rng(4).permutation(l)
+        return m, l, t, None, None, 3
     if name == "fast_medium_flat":
         # plain fast path with medium gaps; the medium prices as type 1
         m = ref_model(128, 128, seed=12)
@@ -187,7 +195,7 @@ def random_pairs(seed, sites):
     return np.array([[rng.next_f64() - 0.5, rng.next_f64() - 0.5] for _ in range(sites)])
 
 
-PARITY_CASES = ["plain128", "surface256", "surface1024", "fast_medium_flat", "fast_medium_typed",
+PARITY_CASES = ["plain128", "surface256", "surface1024", "scattered1024", "fast_medium_flat", "fast_medium_typed",
                 "fast_lone_typed", "ref64", "medium_gaps", "rect64x32",
                 "von_neumann", "float_energies", "volume3d_o2", "volume3d_o3", "volume3d_o1",
                 "connected", "connected3d", "chemotaxis_ramp", "secreting", "length",
