@@ -348,35 +348,57 @@ ghip_status run_sweep_pass(ghip_sim *s, int colour) {
     return GHIP_OK;
 }
 
+// One chunk of `take` substeps of species k (the exchange fused into the
+// species' last chunk), reading the current buffer and writing the other.
+FieldArgs field_chunk(const ghip_sim *s, uint32_t k, int take, bool last) {
+    FieldArgs f{};
+    f.in = s->field[k][s->field_cur[k]];
+    f.out = s->field[k][1 - s->field_cur[k]];
+    f.labels = s->labels[s->lab_cur];
+    f.tables = s->tables;
+    f.w = s->W;
+    f.h = s->H;
+    f.d = s->D;
+    f.ix_log2 = s->fx_log2;
+    f.iy_log2 = s->fy_log2;
+    f.iz = s->fz;
+    f.steps = take;
+    f.exchange = last ? 1 : 0;
+    f.species = (int)k;
+    f.dcoef = s->fd[k];
+    f.kcoef = s->fk[k];
+    f.tiles_x = s->f_tiles_x;
+    f.tiles_y = s->f_tiles_y;
+    return f;
+}
+
 ghip_status run_fields(ghip_sim *s) {
     TimedLaunch t{};
     HIP_TRY(s->begin_timed(1, &t));
-    for (uint32_t k = 0; k < s->m.n_species; ++k) {
-        int left = s->substeps[k];
-        while (left > 0) {
-            int take = std::min(left, s->f_kmax);
-            left -= take;
-            FieldArgs f{};
-            f.in = s->field[k][s->field_cur[k]];
-            f.out = s->field[k][1 - s->field_cur[k]];
-            f.labels = s->labels[s->lab_cur];
-            f.tables = s->tables;
-            f.w = s->W;
-            f.h = s->H;
-            f.d = s->D;
-            f.ix_log2 = s->fx_log2;
-            f.iy_log2 = s->fy_log2;
-            f.iz = s->fz;
-            f.steps = take;
-            f.exchange = left == 0 ? 1 : 0;
-            f.species = (int)k;
-            f.dcoef = s->fd[k];
-            f.kcoef = s->fk[k];
-            f.tiles_x = s->f_tiles_x;
-            f.tiles_y = s->f_tiles_y;
-            HIP_TRY(launch_field(f, s->d3, s->f_blocks, s->stream));
+    const uint32_t S = s->m.n_species;
+    int left[kMaxSpecies] = {0, 0, 0, 0};
+    for (uint32_t k = 0; k < S; ++k) left[k] = s->substeps[k];
+    // planes of at least one 64 x 32 tile: the species whose next chunk has
+    // the same length step together in one launch
+    const bool planes = field_planes_apply(s->W, s->H, s->d3) && s->fx_log2 == 6 && s->fy_log2 == 5;
+    for (;;) {
+        int take = 0;
+        for (uint32_t k = 0; k < S && take == 0; ++k)
+            if (left[k] > 0) take = std::min(left[k], s->f_kmax);
+        if (take == 0) break;
+        FieldPlanes fp{};
+        for (uint32_t k = 0; k < S; ++k) {
+            if (left[k] <= 0 || std::min(left[k], s->f_kmax) != take) continue;
+            left[k] -= take;
+            const FieldArgs f = field_chunk(s, k, take, left[k] == 0);
+            if (planes) {
+                fp.sp[fp.n++] = f;
+            } else {
+                HIP_TRY(launch_field(f, s->d3, s->f_blocks, s->stream));
+            }
             s->field_cur[k] = 1 - s->field_cur[k];
         }
+        if (planes) HIP_TRY(launch_field_planes(fp, s->stream));
     }
     HIP_TRY(s->end_timed(t));
     return GHIP_OK;

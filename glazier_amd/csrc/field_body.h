@@ -88,4 +88,63 @@ __device__ __forceinline__ void field_body(const FieldArgs &a, const int b) {
     }
 }
 
+// The same step on a plane with the 64 x 32 interior tile and the substep
+// count fixed at compile time, so every LDS coordinate divides by a constant;
+// the lattice is at least one tile on each side, so the halo wraps with one
+// compare. Same operations in the same order as field_body, so the same bits.
+constexpr int kFieldIX = 64, kFieldIY = 32;
+
+template <int K>
+__device__ __forceinline__ void field_body_2d(const FieldArgs &a, const int b, float *u, float *v) {
+    constexpr int LX = kFieldIX + 2 * K, LY = kFieldIY + 2 * K, total = LX * LY;
+    const int tx = b % a.tiles_x, ty = b / a.tiles_x;
+    const int x0 = tx * kFieldIX - K, y0 = ty * kFieldIY - K;
+
+    for (int q = threadIdx.x; q < total; q += blockDim.x) {
+        const int lx = q % LX, ly = q / LX;
+        int gx = x0 + lx, gy = y0 + ly;
+        gx += gx < 0 ? a.w : 0;
+        gx -= gx >= a.w ? a.w : 0;
+        gy += gy < 0 ? a.h : 0;
+        gy -= gy >= a.h ? a.h : 0;
+        u[q] = a.in[(size_t)gy * a.w + gx];
+    }
+    __syncthreads();
+
+#pragma unroll
+    for (int s = 1; s <= K; ++s) {
+        for (int q = threadIdx.x; q < total; q += blockDim.x) {
+            const int lx = q % LX, ly = q / LX;
+            if (lx < s || lx >= LX - s || ly < s || ly >= LY - s) continue;
+            const float here = u[q];
+            float sum = u[q + 1] + u[q - 1];
+            sum = sum + u[q + LX];
+            sum = sum + u[q - LX];
+            v[q] = (here + a.dcoef * (sum - 4.0f * here)) - a.kcoef * here;
+        }
+        __syncthreads();
+        float *t = u;
+        u = v;
+        v = t;
+    }
+
+    for (int q = threadIdx.x; q < kFieldIX * kFieldIY; q += blockDim.x) {
+        const int ix = q % kFieldIX, iy = q / kFieldIX;
+        const int gx = tx * kFieldIX + ix, gy = ty * kFieldIY + iy;
+        if (gx >= a.w || gy >= a.h) continue;
+        const size_t site = (size_t)gy * a.w + gx;
+        float val = u[(iy + K) * LX + ix + K];
+        if (a.exchange) {
+            const uint32_t word = a.labels[site];
+            if (label_id(word) != 0) {
+                const uint32_t t = label_type(word);
+                float w2 = val + a.tables->sec[t][a.species];
+                w2 = w2 - a.tables->up[t][a.species] * w2;
+                val = w2 < 0.0f ? 0.0f : w2;
+            }
+        }
+        a.out[site] = val;
+    }
+}
+
 }  // namespace ghip

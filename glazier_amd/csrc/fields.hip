@@ -9,6 +9,17 @@ __global__ __launch_bounds__(256) void field_kernel(FieldArgs a) {
     field_body<D3>(a, blockIdx.x);
 }
 
+// Every species of a plane in one launch, K substeps each on 64 x 32 tiles:
+// block b takes tile b % tiles of species b / tiles.
+template <int K>
+__global__ __launch_bounds__(256) void field_planes_kernel(FieldPlanes p) {
+    constexpr int total = (kFieldIX + 2 * K) * (kFieldIY + 2 * K);
+    __shared__ float u[total], v[total];
+    const int tiles = p.sp[0].tiles_x * p.sp[0].tiles_y;
+    const int s = (int)blockIdx.x / tiles;
+    field_body_2d<K>(p.sp[s], (int)blockIdx.x - s * tiles, u, v);
+}
+
 // Every site's memory steps down by one, floored at zero (activity_decay,
 // glazier-cuda/src/lib.rs:426-432; Activity::decay, motility.rs:39-45).
 __global__ void activity_decay_kernel(float *act, size_t n) {
@@ -31,6 +42,24 @@ hipError_t launch_field(const FieldArgs &a, bool d3, int blocks, hipStream_t str
         hipLaunchKernelGGL(field_kernel<true>, dim3(blocks), dim3(256), lds, stream, a);
     else
         hipLaunchKernelGGL(field_kernel<false>, dim3(blocks), dim3(256), lds, stream, a);
+    return hipGetLastError();
+}
+
+bool field_planes_apply(int w, int h, bool d3) { return !d3 && w >= kFieldIX && h >= kFieldIY; }
+
+hipError_t launch_field_planes(const FieldPlanes &p, hipStream_t stream) {
+    const FieldArgs &f = p.sp[0];
+    if (p.n < 1 || p.n > kMaxSpecies || !field_planes_apply(f.w, f.h, false) || f.ix_log2 != 6 ||
+        f.iy_log2 != 5)
+        return hipErrorInvalidValue;
+    const int blocks = f.tiles_x * f.tiles_y * p.n;
+    switch (f.steps) {
+        case 1: hipLaunchKernelGGL(field_planes_kernel<1>, dim3(blocks), dim3(256), 0, stream, p); break;
+        case 2: hipLaunchKernelGGL(field_planes_kernel<2>, dim3(blocks), dim3(256), 0, stream, p); break;
+        case 3: hipLaunchKernelGGL(field_planes_kernel<3>, dim3(blocks), dim3(256), 0, stream, p); break;
+        case 4: hipLaunchKernelGGL(field_planes_kernel<4>, dim3(blocks), dim3(256), 0, stream, p); break;
+        default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 

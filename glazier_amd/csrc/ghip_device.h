@@ -150,6 +150,13 @@ struct FieldArgs {
     int tiles_x, tiles_y;
 };
 
+// The field step of every species of a plane in one launch: sp[0..n) with
+// the same lattice, tile grid and substep count.
+struct FieldPlanes {
+    FieldArgs sp[kMaxSpecies];
+    int n;
+};
+
 // Per-label bookkeeping launches (recounts, anchors, division/death).
 struct CellArgs {
     uint32_t *labels;      // packed words
@@ -187,6 +194,10 @@ hipError_t launch_sweep_plain2d_resident(const PlainResidentArgs &r, const Plain
                                          hipStream_t stream);
 hipError_t launch_field(const FieldArgs &a, bool d3, int blocks, hipStream_t stream);
 size_t field_lds_bytes(const FieldArgs &a, bool d3);
+// Whether a w x h lattice takes launch_field_planes (a plane of at least one
+// 64 x 32 tile).
+bool field_planes_apply(int w, int h, bool d3);
+hipError_t launch_field_planes(const FieldPlanes &p, hipStream_t stream);
 hipError_t launch_activity_decay(float *act, size_t n, hipStream_t stream);
 // Each label's length minus its type's target length in the snapshot
 // a + e (the length term's `before`, the same for every attempt of a pass).
