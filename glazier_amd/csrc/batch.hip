@@ -141,7 +141,20 @@ __global__ __launch_bounds__(256) void batch_field_kernel(BatchArgs b, WorkList 
     const uint32_t g = find_entry(w, blockIdx.x);
     const FieldArgs &f = b.cur_field[w.members[g]];
     if (f.steps == 0) return;
-    field_body<D3>(f, (int)(blockIdx.x - w.first[g]));
+    const int blk = (int)(blockIdx.x - w.first[g]);
+    if (!D3 && f.ix_log2 == 6 && f.iy_log2 == 5 && field_planes_apply(f.w, f.h, false)) {
+        // a plane of at least one 64 x 32 tile takes the single-lattice
+        // launch's plane body (the dynamic LDS holds its two buffers)
+        extern __shared__ float fl[];
+        switch (f.steps) {
+            case 1: field_body_2d<1>(f, blk, fl, fl + (kFieldIX + 2) * (kFieldIY + 2)); return;
+            case 2: field_body_2d<2>(f, blk, fl, fl + (kFieldIX + 4) * (kFieldIY + 4)); return;
+            case 3: field_body_2d<3>(f, blk, fl, fl + (kFieldIX + 6) * (kFieldIY + 6)); return;
+            case 4: field_body_2d<4>(f, blk, fl, fl + (kFieldIX + 8) * (kFieldIY + 8)); return;
+            default: break;
+        }
+    }
+    field_body<D3>(f, blk);
 }
 
 // ---------------------------------------------------------------- cells

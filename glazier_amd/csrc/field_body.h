@@ -92,7 +92,6 @@ __device__ __forceinline__ void field_body(const FieldArgs &a, const int b) {
 // count fixed at compile time, so every LDS coordinate divides by a constant;
 // the lattice is at least one tile on each side, so the halo wraps with one
 // compare. Same operations in the same order as field_body, so the same bits.
-constexpr int kFieldIX = 64, kFieldIY = 32;
 
 template <int K>
 __device__ __forceinline__ void field_body_2d(const FieldArgs &a, const int b, float *u, float *v) {

@@ -45,8 +45,6 @@ hipError_t launch_field(const FieldArgs &a, bool d3, int blocks, hipStream_t str
     return hipGetLastError();
 }
 
-bool field_planes_apply(int w, int h, bool d3) { return !d3 && w >= kFieldIX && h >= kFieldIY; }
-
 hipError_t launch_field_planes(const FieldPlanes &p, hipStream_t stream) {
     const FieldArgs &f = p.sp[0];
     if (p.n < 1 || p.n > kMaxSpecies || !field_planes_apply(f.w, f.h, false) || f.ix_log2 != 6 ||

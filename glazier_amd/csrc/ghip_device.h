@@ -194,9 +194,12 @@ hipError_t launch_sweep_plain2d_resident(const PlainResidentArgs &r, const Plain
                                          hipStream_t stream);
 hipError_t launch_field(const FieldArgs &a, bool d3, int blocks, hipStream_t stream);
 size_t field_lds_bytes(const FieldArgs &a, bool d3);
-// Whether a w x h lattice takes launch_field_planes (a plane of at least one
-// 64 x 32 tile).
-bool field_planes_apply(int w, int h, bool d3);
+// The plane field body's interior tile; launch_field_planes (and the batch's
+// field launch) take it on a plane of at least one such tile.
+constexpr int kFieldIX = 64, kFieldIY = 32;
+__host__ __device__ inline bool field_planes_apply(int w, int h, bool d3) {
+    return !d3 && w >= kFieldIX && h >= kFieldIY;
+}
 hipError_t launch_field_planes(const FieldPlanes &p, hipStream_t stream);
 hipError_t launch_activity_decay(float *act, size_t n, hipStream_t stream);
 // Each label's length minus its type's target length in the snapshot
