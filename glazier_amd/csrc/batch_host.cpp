@@ -521,7 +521,10 @@ ghip_status ghip_batch_create(ghip_sim *const *sims, size_t n, ghip_batch **out)
     }
     ghip_status st = rebuild(b);
     if (st != GHIP_OK) return cleanup(st);
-    for (ghip_sim *s : b->sims) s->batch = b;
+    for (ghip_sim *s : b->sims) {
+        s->batch = b;
+        s->ever_batched = true;  // the batch's passes do not keep pass_flags
+    }
     *out = b;
     return GHIP_OK;
 }

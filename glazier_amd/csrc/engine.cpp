@@ -329,6 +329,10 @@ ghip_status run_sweep_pass(ghip_sim *s, int colour) {
     a.fold_per_block = (uint32_t)((words + s->blocks - 1) / s->blocks);
     a.xcd_chunk = (s->blocks % 8 == 0) ? s->blocks / 8 : 0;
     a.reverse = (int)(p & 1);
+    if (s->quiet_skip && !s->ever_batched) {
+        a.pass_flags = s->pass_flags;
+        a.pass_slot = (uint32_t)(p & 3);
+    }
     TimedLaunch t{};
     HIP_TRY(s->begin_timed(0, &t));
     if (s->sweep_flags & kLength)
@@ -550,6 +554,8 @@ ghip_status ghip_create_nematic(const ghip_model *model, const uint32_t *labels,
     cap = std::min<uint64_t>(cap, kIdMask - 1);
     s->cap = (uint32_t)cap;
     s->resident = s->plain_physics && m.n_species == 0 && plain2d_resident_fits(s->W, s->H, s->cap);
+    s->quiet_skip = s->plain_fast && !s->resident && !s->tracks_moments && !s->has_pop && !s->has_mot &&
+                    !s->has_nem && m.n_species == 0;
     for (uint32_t k = 0; k < m.n_species; ++k) {
         s->substeps[k] = substeps_for(m, k);
         s->fd[k] = (float)(m.species[k].diffusion / (double)s->substeps[k]);
@@ -596,6 +602,13 @@ ghip_status ghip_create_nematic(const ghip_model *model, const uint32_t *labels,
         CREATE_TRY(hipMemcpy(s->nematic, nq.data(), N * 2 * 4, hipMemcpyHostToDevice));
     }
     CREATE_TRY(hipMalloc(&s->flags, 16));
+    if (s->quiet_skip) {
+        // every pass so far counts as having accepted copies (the second
+        // record buffer does not hold the first's values yet)
+        const uint32_t ones[4] = {1, 1, 1, 1};
+        CREATE_TRY(hipMalloc(&s->pass_flags, sizeof ones));
+        CREATE_TRY(hipMemcpy(s->pass_flags, ones, sizeof ones, hipMemcpyHostToDevice));
+    }
     CREATE_TRY(hipMalloc(&s->type_copy, (size_t)cap * 4));
     CREATE_TRY(hipMalloc(&s->anchor_copy, (size_t)cap * 3 * 4));
     CREATE_TRY(hipMalloc(&s->n_labels_dev, 4));

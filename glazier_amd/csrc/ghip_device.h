@@ -88,6 +88,9 @@ struct SweepArgs {
     uint32_t fold_per_block;
     int xcd_chunk;               // tiles per XCD in the remap, 0 for none
     int reverse;                 // walk the tiles in reverse order (every other pass)
+    uint32_t *pass_flags;        // plain pass: four words, word p % 4 nonzero once pass p accepts
+                                 // a copy (or unknown); null to take every pass whole
+    uint32_t pass_slot;          // this pass's index mod 4
     int tiles_x, tiles_y, tiles_z;
     int off[26][3];
 };

@@ -91,6 +91,11 @@ struct ghip_sim {
     bool plain_physics = false;  // the medium, one cell type, integer contact/volume/surface, Moore plane
     bool plain_fast = false;     // the plain-monolayer pass (sweep_plain.hip) applies
     bool resident = false;       // the whole lattice steps inside one workgroup
+    // a step is its plain streaming passes and nothing else, so the passes
+    // can tell from pass_flags when the last two accepted nothing anywhere
+    bool quiet_skip = false;
+    bool ever_batched = false;   // stepped by a batch at some point: pass_flags no longer apply
+    uint32_t *pass_flags = nullptr;
     int fx_log2 = 6, fy_log2 = 5, fz = 1, f_tiles_x = 1, f_tiles_y = 1, f_blocks = 1, f_kmax = 4;
 
     bool timing = false;         // every launch bracketed by its own events
@@ -112,7 +117,7 @@ struct ghip_sim {
         void *bufs[] = {labels[0], labels[1], act[0], act[1], field_base[0], field_base[1], A[0],
                         A[1], Dl[0], Dl[1], Dl[2], anchor, ltype, scratch, daughter, plan, geo,
                         flags, type_copy, anchor_copy, n_labels_dev, tables, accepted, traj, len_before,
-                        nematic};
+                        nematic, pass_flags};
         for (void *p : bufs)
             if (p) (void)hipFree(p);
         for (auto *list : {&pending, &spare})

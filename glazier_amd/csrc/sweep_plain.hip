@@ -192,7 +192,7 @@ struct PlainTile {
 
     template <int NT>
     __device__ __forceinline__ uint32_t phase_b(const SweepArgs &a, const PlainParams &p, const uint32_t *thr_s,
-                                                const uint16_t *queue, uint32_t nq, bool has_medium,
+                                                const uint16_t *queue, uint32_t nq, bool has_medium, bool e_zero,
                                                 uint32_t &d_old, uint32_t &d_new) const {
         const uint32_t cw = (uint32_t)a.cell_words;
         uint32_t my_acc = 0;
@@ -206,10 +206,15 @@ struct PlainTile {
             const uint32_t old = tile[tl];
             const uint32_t nw = picked(ph, tl);
             const uint32_t oid = label_id(old), nid = label_id(nw);
-            // the four gathers go out together, before anything waits on them;
-            // label 0's record is real memory whose value is never used
-            const int64_t ao = a.a_cur[oid * cw], eo = a.e_prev[oid * cw];
-            const int64_t an = a.a_cur[nid * cw], en = a.e_prev[nid * cw];
+            // the gathers go out together, before anything waits on them;
+            // label 0's record is real memory whose value is never used. When
+            // the previous pass accepted nothing its deltas are all zero.
+            const int64_t ao = a.a_cur[oid * cw], an = a.a_cur[nid * cw];
+            int64_t eo = 0, en = 0;
+            if (!e_zero) {
+                eo = a.e_prev[oid * cw];
+                en = a.e_prev[nid * cw];
+            }
             const uint32_t n0 = tile[tl - kPitch - 1], n1 = tile[tl - kPitch], n2 = tile[tl - kPitch + 1];
             const uint32_t n3 = tile[tl - 1], n4 = tile[tl + 1];
             const uint32_t n5 = tile[tl + kPitch - 1], n6 = tile[tl + kPitch], n7 = tile[tl + kPitch + 1];
@@ -308,6 +313,19 @@ __global__ __launch_bounds__(NT) void sweep_plain2d_kernel(SweepArgs a, PlainPar
         acc_block = 0;
     }
 
+    // Passes p - 1 and p - 2 accepted nothing anywhere: the previous pass's
+    // deltas are all zero (e_zero), and the fold is the identity as well
+    // (fold_idle): the record buffer it would write already holds the
+    // snapshot, and the delta buffer it would clear is already clear. Pass p
+    // clears the word of pass p + 1, which no pass p reads.
+    bool e_zero = false, fold_idle = false;
+    if (a.pass_flags) {
+        const uint32_t f1 = a.pass_flags[(a.pass_slot + 3) & 3u], f2 = a.pass_flags[(a.pass_slot + 2) & 3u];
+        e_zero = f1 == 0;
+        fold_idle = e_zero && f2 == 0;
+        if (blockIdx.x == 0 && tid == 0) a.pass_flags[(a.pass_slot + 1) & 3u] = 0;
+    }
+
     // ---- issue every global read of the workgroup before the first wait
     uint4 v[kLoads];
     int vdst[kLoads];
@@ -342,7 +360,7 @@ __global__ __launch_bounds__(NT) void sweep_plain2d_kernel(SweepArgs a, PlainPar
     if (tid < (int)p.n_thr) tv = p.thr[tid];
     const uint32_t words = *a.n_labels * (uint32_t)a.cell_words;
     const uint32_t flo = blockIdx.x * a.fold_per_block;
-    const uint32_t fhi = min(flo + a.fold_per_block, words);
+    const uint32_t fhi = fold_idle ? flo : min(flo + a.fold_per_block, words);
     const uint32_t fi = flo + tid;
     int64_t fa = 0, fe = 0;
     if (fi < fhi) {
@@ -378,14 +396,17 @@ __global__ __launch_bounds__(NT) void sweep_plain2d_kernel(SweepArgs a, PlainPar
     pt.template phase_a<NT>(a, queue, &qcount);
     __syncthreads();
     uint32_t d_old, d_new;
-    const uint32_t my_acc = pt.template phase_b<NT>(a, p, thr_s, queue, qcount, has_medium, d_old, d_new);
+    const uint32_t my_acc = pt.template phase_b<NT>(a, p, thr_s, queue, qcount, has_medium, e_zero, d_old, d_new);
     if (my_acc) atomicAdd(&acc_block, my_acc);
     __syncthreads();
     // every queue entry is read; a tile with accepted copies merges their
     // record changes (a frozen tile takes no further barrier)
     const uint32_t acc = acc_block;
     if (acc != 0) pt.template merge_deltas<NT>(a, queue, d_old, d_new);
-    if (tid == 0 && acc != 0) atomicAdd(a.accepted + (blockIdx.x & (kAcceptSlots - 1)), (unsigned long long)acc);
+    if (tid == 0 && acc != 0) {
+        atomicAdd(a.accepted + (blockIdx.x & (kAcceptSlots - 1)), (unsigned long long)acc);
+        if (a.pass_flags) a.pass_flags[a.pass_slot] = 1u;
+    }
 }
 
 // ---- the resident pass: a small lattice stepped whole inside one workgroup
