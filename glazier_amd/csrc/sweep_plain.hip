@@ -49,15 +49,17 @@ __device__ __forceinline__ uint32_t unequal8(uint32_t v, uint32_t n0, uint32_t n
 // The decision on one target whose label would change from `old` to `nw`:
 // contact gained minus lost from neighbour counts (exact with integer
 // energies), volume and surface against each label's pass-start record
-// (volume * 2^32 + surface), then the integral acceptance table. Shared by
-// the streaming pass and the resident one, so both decide identically.
+// (volume * 2^32 + surface), then the integral acceptance table. The
+// uniform's hash round (of the target's pick round `ph`) is taken only when
+// the table decides, 0 < dH < its length. Shared by the streaming pass and
+// the resident one, so both decide identically.
 struct PlainMove {
     bool acc;
     int ds_old, ds_new;
 };
 
-__device__ __forceinline__ PlainMove plain_move(const PlainParams &p, const uint32_t *thr_s, uint32_t r24,
-                                                uint32_t old, uint32_t nw, uint32_t n0, uint32_t n1,
+__device__ __forceinline__ PlainMove plain_move(const PlainParams &p, const uint32_t *thr_s, uint32_t key2,
+                                                uint32_t ph, uint32_t old, uint32_t nw, uint32_t n0, uint32_t n1,
                                                 uint32_t n2, uint32_t n3, uint32_t n4, uint32_t n5,
                                                 uint32_t n6, uint32_t n7, int64_t pv_old, int64_t pv_new,
                                                 bool has_medium) {
@@ -106,7 +108,9 @@ __device__ __forceinline__ PlainMove plain_move(const PlainParams &p, const uint
         const float dsf = (float)mv.ds_new;
         dh = dh + p.lam_s * (dsf * (2.0f * (s - p.tgt_s) + dsf));
     }
-    mv.acc = !emptying && (dh <= 0.0f || (dh < (float)p.n_thr && r24 < thr_s[(int)dh]));
+    bool acc = dh <= 0.0f;
+    if (!acc && dh < (float)p.n_thr) acc = (site_draw(key2, ph) >> 8) < thr_s[(int)dh];
+    mv.acc = !emptying && acc;
     return mv;
 }
 }  // namespace
@@ -181,8 +185,8 @@ struct PlainTile {
 
     // Phase B: price the nq queued targets, write the accepted copies'
     // labels, and return this lane's accepted count. It redraws each
-    // target's pick round (cheaper than holding it in LDS), takes the
-    // uniform's round, and issues the four per-label gathers together. The
+    // target's pick round (cheaper than holding it in LDS) and issues the
+    // per-label gathers together. The
     // record changes of a lane's first entry (queue entry tid) leave in
     // d_old and d_new as id | (ds + 8) / 2 << 28 (the surface change ds is
     // even, -8..8; the volume changes by -1 and +1), or kEmpty, for
@@ -202,7 +206,6 @@ struct PlainTile {
             int tl;
             target(a, queue[qi], &site, &tl);
             const uint32_t ph = site_pick(a.key, site);
-            const uint32_t r24 = site_draw(a.key2, ph) >> 8;
             const uint32_t old = tile[tl];
             const uint32_t nw = picked(ph, tl);
             const uint32_t oid = label_id(old), nid = label_id(nw);
@@ -219,7 +222,8 @@ struct PlainTile {
             const uint32_t n3 = tile[tl - 1], n4 = tile[tl + 1];
             const uint32_t n5 = tile[tl + kPitch - 1], n6 = tile[tl + kPitch], n7 = tile[tl + kPitch + 1];
             const PlainMove mv =
-                plain_move(p, thr_s, r24, old, nw, n0, n1, n2, n3, n4, n5, n6, n7, ao + eo, an + en, has_medium);
+                plain_move(p, thr_s, a.key2, ph, old, nw, n0, n1, n2, n3, n4, n5, n6, n7, ao + eo, an + en,
+                           has_medium);
             if (!mv.acc) continue;
             a.labels[site] = nw;
             if (qi < (uint32_t)NT) {
@@ -512,8 +516,7 @@ __global__ __launch_bounds__(kResThreads) void sweep_plain2d_resident_kernel(Pla
                 const uint32_t nw = k == 0 ? n0 : k == 1 ? n1 : k == 2 ? n2 : k == 3 ? n3
                                   : k == 4 ? n4 : k == 5 ? n5 : k == 6 ? n6 : n7;
                 const uint32_t oid = label_id(old), nid = label_id(nw);
-                const uint32_t r24 = site_draw(key2, ph) >> 8;
-                const PlainMove mv = plain_move(p, thr_s, r24, old, nw, n0, n1, n2, n3, n4, n5, n6, n7,
+                const PlainMove mv = plain_move(p, thr_s, key2, ph, old, nw, n0, n1, n2, n3, n4, n5, n6, n7,
                                                 rec[oid], rec[nid], p.needs_medium != 0);
                 if (!mv.acc) continue;
                 lat[site] = nw;
