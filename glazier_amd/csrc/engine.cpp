@@ -333,6 +333,8 @@ ghip_status run_sweep_pass(ghip_sim *s, int colour) {
         a.pass_flags = s->pass_flags;
         a.pass_slot = (uint32_t)(p & 3);
     }
+    // more tiles than four workgroups on every CU: more than one round
+    a.merge_deltas = s->blocks > 4 * s->cus ? 1 : 0;
     TimedLaunch t{};
     HIP_TRY(s->begin_timed(0, &t));
     if (s->sweep_flags & kLength)
@@ -574,6 +576,11 @@ ghip_status ghip_create_nematic(const ghip_model *model, const uint32_t *labels,
     } while (0)
 
     CREATE_TRY(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+    {
+        int dev = 0;
+        CREATE_TRY(hipGetDevice(&dev));
+        CREATE_TRY(hipDeviceGetAttribute(&s->cus, hipDeviceAttributeMultiprocessorCount, dev));
+    }
     CREATE_TRY(hipMalloc(&s->labels[0], N * 4));
     if (s->has_mot) CREATE_TRY(hipMalloc(&s->labels[1], N * 4));
     CREATE_TRY(hipMalloc(&s->act[0], N * 4));

@@ -91,6 +91,7 @@ struct SweepArgs {
     uint32_t *pass_flags;        // plain pass: four words, word p % 4 nonzero once pass p accepts
                                  // a copy (or unknown); null to take every pass whole
     uint32_t pass_slot;          // this pass's index mod 4
+    int merge_deltas;            // plain pass: merge a tile's record changes in LDS first
     int tiles_x, tiles_y, tiles_z;
     int off[26][3];
 };

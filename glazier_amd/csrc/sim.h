@@ -96,6 +96,7 @@ struct ghip_sim {
     bool quiet_skip = false;
     bool ever_batched = false;   // stepped by a batch at some point: pass_flags no longer apply
     uint32_t *pass_flags = nullptr;
+    int cus = 0;                 // compute units of the device
     int fx_log2 = 6, fy_log2 = 5, fz = 1, f_tiles_x = 1, f_tiles_y = 1, f_blocks = 1, f_kmax = 4;
 
     bool timing = false;         // every launch bracketed by its own events

@@ -190,8 +190,9 @@ struct PlainTile {
     // record changes of a lane's first entry (queue entry tid) leave in
     // d_old and d_new as id | (ds + 8) / 2 << 28 (the surface change ds is
     // even, -8..8; the volume changes by -1 and +1), or kEmpty, for
-    // merge_deltas; those of later entries (a tile with more than NT queued
-    // targets) go straight to the records.
+    // merge_deltas when the launch merges; those of later entries (a tile
+    // with more than NT queued targets), and all of them when it does not,
+    // go straight to the records.
     static constexpr uint32_t kEmpty = 0xFFFFFFFFu;
 
     template <int NT>
@@ -226,7 +227,7 @@ struct PlainTile {
                            has_medium);
             if (!mv.acc) continue;
             a.labels[site] = nw;
-            if (qi < (uint32_t)NT) {
+            if (a.merge_deltas && qi < (uint32_t)NT) {
                 if (oid != 0) d_old = oid | (uint32_t)((mv.ds_old + 8) >> 1) << kTypeShift;
                 if (nid != 0) d_new = nid | (uint32_t)((mv.ds_new + 8) >> 1) << kTypeShift;
             } else {
@@ -364,7 +365,7 @@ __global__ __launch_bounds__(NT) void sweep_plain2d_kernel(SweepArgs a, PlainPar
     if (tid < (int)p.n_thr) tv = p.thr[tid];
     const uint32_t words = *a.n_labels * (uint32_t)a.cell_words;
     const uint32_t flo = blockIdx.x * a.fold_per_block;
-    const uint32_t fhi = fold_idle ? flo : min(flo + a.fold_per_block, words);
+    const uint32_t fhi = min(flo + a.fold_per_block, words);
     const uint32_t fi = flo + tid;
     int64_t fa = 0, fe = 0;
     if (fi < fhi) {
@@ -386,11 +387,13 @@ __global__ __launch_bounds__(NT) void sweep_plain2d_kernel(SweepArgs a, PlainPar
         lo = min(lo, hv);
     }
     if (tid < (int)p.n_thr) thr_s[tid] = tv;
-    if (fi < fhi) {
+    // the fold's loads go out with the tile's whatever the flags say (so they
+    // wait on nothing); an idle fold only skips its stores
+    if (!fold_idle && fi < fhi) {
         a.a_next[fi] = fa + fe;
         a.g_zero[fi] = 0;
     }
-    for (uint32_t i = fi + NT; i < fhi; i += NT) {
+    for (uint32_t i = fi + NT; !fold_idle && i < fhi; i += NT) {
         a.a_next[i] = a.a_cur[i] + a.e_prev[i];
         a.g_zero[i] = 0;
     }
@@ -404,9 +407,12 @@ __global__ __launch_bounds__(NT) void sweep_plain2d_kernel(SweepArgs a, PlainPar
     if (my_acc) atomicAdd(&acc_block, my_acc);
     __syncthreads();
     // every queue entry is read; a tile with accepted copies merges their
-    // record changes (a frozen tile takes no further barrier)
+    // record changes (a frozen tile takes no further barrier). The merge pays
+    // where a pass is many rounds of workgroups and the atomics' traffic
+    // counts; on a lattice of one short round its barriers only lengthen the
+    // chain, and the host leaves it off.
     const uint32_t acc = acc_block;
-    if (acc != 0) pt.template merge_deltas<NT>(a, queue, d_old, d_new);
+    if (acc != 0 && a.merge_deltas) pt.template merge_deltas<NT>(a, queue, d_old, d_new);
     if (tid == 0 && acc != 0) {
         atomicAdd(a.accepted + (blockIdx.x & (kAcceptSlots - 1)), (unsigned long long)acc);
         if (a.pass_flags) a.pass_flags[a.pass_slot] = 1u;
