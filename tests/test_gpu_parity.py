@@ -166,3 +166,25 @@ def test_a_4096_sheet_of_many_rounds_matches_the_emulation(engine, oracle_lib, n
     assert gpu.counters()[1] > 0
     gpu.close()
     emu.close()
+
+
+def test_a_large_lattice_keeps_its_books_across_launches(engine, oracle_lib):
+    # the per-step bookkeeping's chain of gated launches at scale: divisions
+    # on the first step, deaths every step, and a compaction once a quarter
+    # of the labels are dead (step 7)
+    from glazier_amd import HipSimulation
+    from tests.cases import ref_model, tiled
+
+    m = ref_model(1024, 1024, seed=41, division_volume=64.0, death_rate=0.05)
+    labels, types = tiled(m)
+    print("[parity] 1024^2 division and death, 8 MCS", flush=True)
+    gpu = HipSimulation(m, labels, types)
+    emu = oracle.EmuSimulation(m, labels, types)
+    for chunk in (1, 5, 2):
+        gpu.step(chunk)
+        emu.step(chunk)
+        assert gpu.n_labels() == emu.n_labels()
+        assert np.array_equal(gpu.labels(), emu.labels())
+        assert np.array_equal(gpu.cell_state()[1:], emu.cell_state()[1:])
+    assert gpu.n_labels() < 26000  # compacted
+    gpu.close()
