@@ -335,6 +335,8 @@ ghip_status run_sweep_pass(ghip_sim *s, int colour) {
     }
     // more tiles than four workgroups on every CU: more than one round
     a.merge_deltas = s->blocks > 4 * s->cus ? 1 : 0;
+    // without division or death the label count is fixed: every record is live
+    a.fold_words = s->has_pop ? 0u : (uint32_t)words;
     TimedLaunch t{};
     HIP_TRY(s->begin_timed(0, &t));
     if (s->sweep_flags & kLength)

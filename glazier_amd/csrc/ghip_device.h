@@ -92,6 +92,8 @@ struct SweepArgs {
                                  // a copy (or unknown); null to take every pass whole
     uint32_t pass_slot;          // this pass's index mod 4
     int merge_deltas;            // plain pass: merge a tile's record changes in LDS first
+    uint32_t fold_words;         // plain pass: record words to fold when the label count is fixed
+                                 // (no division or death), 0 to read it on the device
     int tiles_x, tiles_y, tiles_z;
     int off[26][3];
 };

@@ -154,17 +154,32 @@ struct PlainTile {
     // ballots of the count's three bits.
     template <int NT>
     __device__ __forceinline__ void phase_a(const SweepArgs &a, uint16_t *queue, uint32_t *qcount) const {
+        queue_moves<NT>(draw<NT, 0, kTargets / NT>(a), queue, qcount);
+    }
+
+    // The draws of a lane's targets k in [K0, K1): bit k set when target k
+    // would change. Target k of the lane lies on tile row 2 j + cy, j =
+    // (k NT + tid) / kTRow, and reads the LDS rows either side of it.
+    template <int NT, int K0, int K1>
+    __device__ __forceinline__ uint32_t draw(const SweepArgs &a) const {
+        uint32_t moves = 0;
+#pragma unroll
+        for (int k = K0; k < K1; ++k) {
+            uint32_t site;
+            int tl;
+            target(a, k * NT + (int)threadIdx.x, &site, &tl);
+            moves |= (tile[tl] != picked(site_pick(a.key, site), tl) ? 1u : 0u) << k;
+        }
+        return moves;
+    }
+
+    // The queue entries of the targets `moves` marks: one reservation per
+    // wavefront.
+    template <int NT>
+    __device__ __forceinline__ void queue_moves(uint32_t moves, uint16_t *queue, uint32_t *qcount) const {
         constexpr int kPerLane = kTargets / NT;
         static_assert(kTargets % NT == 0 && kPerLane <= 7, "a lane's count fits three bits");
         const int tid = threadIdx.x;
-        uint32_t moves = 0;
-#pragma unroll
-        for (int k = 0; k < kPerLane; ++k) {
-            uint32_t site;
-            int tl;
-            target(a, k * NT + tid, &site, &tl);
-            moves |= (tile[tl] != picked(site_pick(a.key, site), tl) ? 1u : 0u) << k;
-        }
         const uint32_t cnt = (uint32_t)__builtin_popcount(moves);
         const unsigned long long c0 = __ballot(cnt & 1u), c1 = __ballot(cnt & 2u), c2 = __ballot(cnt & 4u);
         const uint32_t total = (uint32_t)(__builtin_popcountll(c0) + 2 * __builtin_popcountll(c1) +
@@ -186,8 +201,8 @@ struct PlainTile {
     // Phase B: price the nq queued targets, write the accepted copies'
     // labels, and return this lane's accepted count. It redraws each
     // target's pick round (cheaper than holding it in LDS) and issues the
-    // per-label gathers together. The
-    // record changes of a lane's first entry (queue entry tid) leave in
+    // per-label gathers together. The record changes of a lane's first
+    // entry (queue entry tid) leave in
     // d_old and d_new as id | (ds + 8) / 2 << 28 (the surface change ds is
     // even, -8..8; the volume changes by -1 and +1), or kEmpty, for
     // merge_deltas when the launch merges; those of later entries (a tile
@@ -323,35 +338,20 @@ __global__ __launch_bounds__(NT) void sweep_plain2d_kernel(SweepArgs a, PlainPar
     // (fold_idle): the record buffer it would write already holds the
     // snapshot, and the delta buffer it would clear is already clear. Pass p
     // clears the word of pass p + 1, which no pass p reads.
-    bool e_zero = false, fold_idle = false;
-    if (a.pass_flags) {
-        const uint32_t f1 = a.pass_flags[(a.pass_slot + 3) & 3u], f2 = a.pass_flags[(a.pass_slot + 2) & 3u];
-        e_zero = f1 == 0;
-        fold_idle = e_zero && f2 == 0;
-        if (blockIdx.x == 0 && tid == 0) a.pass_flags[(a.pass_slot + 1) & 3u] = 0;
-    }
+    if (a.pass_flags && blockIdx.x == 0 && tid == 0) a.pass_flags[(a.pass_slot + 1) & 3u] = 0;
 
-    // ---- issue every global read of the workgroup before the first wait
-    uint4 v[kLoads];
-    int vdst[kLoads];
-#pragma unroll
-    for (int k = 0; k < kLoads; ++k) {
-        const int idx = k * NT + tid;
-        const int r = idx / kC4, c4 = idx % kC4;
-        vdst[k] = -1;
-        if (r < kRows) {
-            int g = y0 - 1 + r;
-            g += g < 0 ? H : 0;
-            g -= g >= H ? H : 0;
-            v[k] = *reinterpret_cast<const uint4 *>(a.labels + (uint32_t)g * (uint32_t)W + (uint32_t)x0 +
-                                                    4u * (uint32_t)c4);
-            vdst[k] = r * kPitch + 4 + 4 * c4;
-        }
-    }
-    uint32_t hv = 0;
-    int hpos = -1;
-    if (tid < kRows * 2) {
-        const int r = tid >> 1, right = tid & 1;
+    // ---- issue every global read of the workgroup before the first wait, in
+    // the order they are used: the halo words and the acceptance table, the
+    // tile's rows, the fold's words, the pass flags. Every lane issues every
+    // load, with no branch (a lane with nothing to load reads a valid word
+    // and ignores it), so each wait covers exactly the loads it needs: a
+    // wavefront's wait for a load is a wait for every load issued before it.
+    const uint32_t n_labels = *a.n_labels;
+    const uint32_t words = a.fold_words ? a.fold_words : n_labels * (uint32_t)a.cell_words;
+    const bool h_live = tid < kRows * 2;
+    uint32_t hv;
+    {
+        const int r = h_live ? tid >> 1 : 0, right = tid & 1;
         int gy = y0 - 1 + r;
         gy += gy < 0 ? H : 0;
         gy -= gy >= H ? H : 0;
@@ -359,34 +359,79 @@ __global__ __launch_bounds__(NT) void sweep_plain2d_kernel(SweepArgs a, PlainPar
         gx += gx < 0 ? W : 0;
         gx -= gx >= W ? W : 0;
         hv = a.labels[(uint32_t)gy * (uint32_t)W + (uint32_t)gx];
-        hpos = r * kPitch + (right ? 4 + PX : 3);
     }
-    uint32_t tv = 0;
-    if (tid < (int)p.n_thr) tv = p.thr[tid];
-    const uint32_t words = *a.n_labels * (uint32_t)a.cell_words;
+    // a lane with no halo word stores its load into a spare word past the
+    // tile, so the load has an unconditional use and stays where it is
+    // issued. Word kRows * kPitch is the last row's right halo; words +1 and
+    // +2 are spare.
+    constexpr int kSpare = kRows * kPitch + 1;
+    const int hpos = h_live ? (tid >> 1) * kPitch + ((tid & 1) ? 4 + PX : 3) : kSpare + (tid & 1);
+    const uint32_t tv = p.thr[tid < (int)p.n_thr ? tid : 0];
+    uint4 v[kLoads];
+    int vdst[kLoads];
+#pragma unroll
+    for (int k = 0; k < kLoads; ++k) {
+        const int idx = k * NT + tid;
+        const int r = idx / kC4, c4 = idx % kC4;
+        int g = y0 - 1 + (r < kRows ? r : kRows - 1);
+        g += g < 0 ? H : 0;
+        g -= g >= H ? H : 0;
+        v[k] = *reinterpret_cast<const uint4 *>(a.labels + (uint32_t)g * (uint32_t)W + (uint32_t)x0 +
+                                                4u * (uint32_t)c4);
+        vdst[k] = r < kRows ? r * kPitch + 4 + 4 * c4 : -1;
+    }
     const uint32_t flo = blockIdx.x * a.fold_per_block;
     const uint32_t fhi = min(flo + a.fold_per_block, words);
     const uint32_t fi = flo + tid;
-    int64_t fa = 0, fe = 0;
-    if (fi < fhi) {
-        fa = a.a_cur[fi];
-        fe = a.e_prev[fi];
-    }
+    const uint32_t fc = fi < fhi ? fi : 0u;
+    const int64_t fa = a.a_cur[fc], fe = a.e_prev[fc];
+    // Passes p - 1 and p - 2 accepted nothing anywhere: the previous pass's
+    // deltas are all zero (e_zero), and the fold is the identity as well
+    // (fold_idle): the record buffer it would write already holds the
+    // snapshot, and the delta buffer it would clear is already clear. Pass p
+    // cleared the word of pass p + 1 above, which no pass p reads.
+    const uint32_t *flags = a.pass_flags ? a.pass_flags : reinterpret_cast<const uint32_t *>(a.accepted);
+    const uint32_t f1 = flags[(a.pass_slot + 3) & 3u], f2 = flags[(a.pass_slot + 2) & 3u];
 
-    // ---- land them
+    // ---- land them in two parts. The first kL1 loads hold the tile's first
+    // rows; the lane's first kA1 targets read no row beyond them, so they are
+    // drawn while the rest of the tile is still arriving.
+    constexpr int kRowsPerLoad = NT / kC4;
+    static_assert(NT % kC4 == 0, "a load instruction covers whole rows");
+    constexpr int kL1 = (kLoads + 1) / 2;
+    constexpr int kR1 = kL1 * kRowsPerLoad < kRows ? kL1 * kRowsPerLoad : kRows;  // rows in the first part
+    constexpr int kTRowsPerDraw = NT / T::kTRow;  // target rows one draw of every lane covers
+    constexpr int kPerLane = T::kTargets / NT;
+    // draw k covers target rows j < (k + 1) kTRowsPerDraw, and row j reads
+    // LDS rows 2 j + cy .. 2 j + cy + 2 (cy <= 1): the last early draw's rows
+    // end below kR1 when 2 kA1 kTRowsPerDraw + 1 < kR1
+    constexpr int kA0 = (kR1 - 2) / (2 * kTRowsPerDraw);
+    constexpr int kA1 = kA0 < kPerLane ? kA0 : kPerLane;
+    static_assert(kA1 == 0 || 2 * (kA1 * kTRowsPerDraw - 1) + 3 < kR1, "early draws read landed rows only");
     uint32_t lo = 0xFFFFFFFFu;
+    auto land = [&](int k0, int k1) {
 #pragma unroll
-    for (int k = 0; k < kLoads; ++k) {
-        if (vdst[k] >= 0) {
+        for (int k = 0; k < kLoads; ++k) {
+            if (k < k0 || k >= k1 || vdst[k] < 0) continue;
             *reinterpret_cast<uint4 *>(tile + vdst[k]) = v[k];
             if (p.needs_medium) lo = min(lo, min(min(v[k].x, v[k].y), min(v[k].z, v[k].w)));
         }
+    };
+    const PlainTile<PX, PY> pt{tile, x0, y0};
+    uint32_t moves = 0;
+    land(0, kA1 > 0 ? kL1 : kLoads);
+    tile[hpos] = hv;
+    lo = min(lo, h_live ? hv : 0xFFFFFFFFu);
+    // entries past the table's length are never read; lanes past the LDS
+    // table store into a spare word (an unconditional use, as for the halo)
+    *(tid < kPThr ? &thr_s[tid] : &tile[kSpare + (tid & 1)]) = tv;
+    if constexpr (kA1 > 0) {
+        __syncthreads();
+        moves = pt.template draw<NT, 0, kA1>(a);
+        land(kL1, kLoads);
     }
-    if (hpos >= 0) {
-        tile[hpos] = hv;
-        lo = min(lo, hv);
-    }
-    if (tid < (int)p.n_thr) thr_s[tid] = tv;
+    const bool e_zero = a.pass_flags && f1 == 0;
+    const bool fold_idle = e_zero && f2 == 0;
     // the fold's loads go out with the tile's whatever the flags say (so they
     // wait on nothing); an idle fold only skips its stores
     if (!fold_idle && fi < fhi) {
@@ -399,8 +444,8 @@ __global__ __launch_bounds__(NT) void sweep_plain2d_kernel(SweepArgs a, PlainPar
     }
     const bool has_medium = __syncthreads_or(p.needs_medium && lo == 0u) != 0;
 
-    const PlainTile<PX, PY> pt{tile, x0, y0};
-    pt.template phase_a<NT>(a, queue, &qcount);
+    moves |= pt.template draw<NT, kA1, kPerLane>(a);
+    pt.template queue_moves<NT>(moves, queue, &qcount);
     __syncthreads();
     uint32_t d_old, d_new;
     const uint32_t my_acc = pt.template phase_b<NT>(a, p, thr_s, queue, qcount, has_medium, e_zero, d_old, d_new);
