@@ -7,26 +7,34 @@
 //
 //  * a workgroup owns a PX x PY tile (256 x 32 sites with 512 lanes above
 //    512^2 where the lattice divides, 128 x 8 up to 256^2, else 128 x 16,
-//    both with 256 lanes). Every
-//    global read of the workgroup (the PY + 2 tile rows as 16-byte loads,
-//    the halo columns, the acceptance table and its share of the snapshot
-//    fold) is issued before the first wait; rows land in an LDS tile of pitch PX + 4 words, the
-//    interior at column 4 (so each 16-byte LDS write is aligned), the left
-//    halo at column 3 and the right halo one past the interior, which is
-//    column 0 of the next row. At 256 x 32 that keeps the workgroup under
-//    40 KiB of LDS, four per CU, and 2048 tiles of a 4096^2 lattice fill
-//    the chip in exactly two rounds;
-//  * phase A draws every target. A wavefront's 64 targets share one tile row,
-//    so row bases stay in scalar registers; the target's label and its picked
+//    both with 256 lanes), the tiles of each XCD in one contiguous run,
+//    walked backwards on every other pass so a pass starts on the lines the
+//    previous one left in that XCD's L2. Every global read of the
+//    workgroup (halo columns, acceptance table, the PY + 2 tile rows as
+//    16-byte loads, its share of the snapshot fold, the pass flags) is
+//    issued before the first wait, branch-free and in the order it is used.
+//    Rows land in an LDS tile of pitch PX + 4 words, the interior at column
+//    4 (so each 16-byte LDS write is aligned), the left halo at column 3 and
+//    the right halo one past the interior, which is column 0 of the next
+//    row. At 256 x 32 that keeps the workgroup under 40 KiB of LDS, four per
+//    CU, and 2048 tiles of a 4096^2 lattice fill the chip in two rounds;
+//  * phase A draws every target (one hash round: the pick). A wavefront's 64
+//    targets share one tile row; the target's label and its picked
 //    neighbour come from LDS, and the targets whose label would change are
-//    appended to an LDS queue of 16-bit target indices (one LDS atomic per
-//    wavefront). In a confluent sheet most targets have old == new and cost
-//    nothing more;
-//  * phase B prices the queued targets densely: it redraws the target's hash
-//    (cheaper than holding it in LDS, which would cost residency), issues the
-//    four per-label gathers together, and takes the contact sum from
-//    neighbour counts (exact with integer energies); the medium count is
-//    skipped for tiles that hold no medium.
+//    appended to an LDS queue of 16-bit target indices (one reservation per
+//    wavefront). The targets that read only the tile's first rows are drawn
+//    while the last rows still arrive. In a confluent sheet most targets
+//    have old == new and cost nothing more;
+//  * phase B prices the queued targets densely: it redraws the pick round
+//    (cheaper than holding it in LDS, which would cost residency), issues
+//    the per-label gathers together, takes the contact sum from neighbour
+//    counts (exact with integer energies) and the uniform's hash round only
+//    when the acceptance table decides; the medium count is skipped for
+//    tiles that hold no medium. On lattices of several rounds a tile merges
+//    its accepted copies' record changes per label in LDS before the global
+//    atomics;
+//  * a pass whose two predecessors accepted nothing anywhere (the pass
+//    flags) reads no deltas and stores no fold.
 //
 // Larger tiles carry more loads per workgroup lifetime and amortise the
 // fixed per-workgroup work; they matter most once copies are accepted.
