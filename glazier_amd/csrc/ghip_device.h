@@ -32,7 +32,6 @@ struct DeviceTables {
     double death[kMaxTypes];
     uint32_t connected[kMaxTypes];
     uint32_t piece2d[8];      // bit m: the planar ring mask m is one piece
-    uint32_t adj3d[26];       // ring positions adjacent to each position
     uint32_t thr[kMaxThr];
     uint32_t n_thr;
     float accept_c;

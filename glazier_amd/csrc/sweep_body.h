@@ -4,7 +4,7 @@
 // the serial attempt (glazier-core/src/cpu.rs:432-517), laid out for CDNA4:
 //
 //  * a workgroup owns a tile of 2TX x 2TY (x 2TZ) sites; it stages the tile's
-//    labels plus a Moore halo in LDS with whole-row loads, so every target
+//    labels plus a Moore halo in LDS with all lanes' loads in flight, so every target
 //    reads its neighbourhood from LDS instead of strided global gathers;
 //  * one lane per target: TX targets along x per row, so a 64-lane wavefront
 //    covers one target row of a 128-site tile row;
@@ -51,24 +51,36 @@ __device__ __forceinline__ void ring_offset(int k, bool d3, int *dx, int *dy, in
     *dx = q % 3 - 1;
 }
 
-// Whether the ring positions set in `present` form one piece.
+// The 3 x 3 x 3 cube as 27 bits, bit (dz+1)*9 + (dy+1)*3 + (dx+1): every
+// set position with its 26 neighbours. The Moore (Chebyshev) neighbourhood
+// is the product of the three axes' {-1, 0, 1}, so the dilation is one shift
+// pair per axis, each masked so no bit wraps into the next row or plane.
+__device__ __forceinline__ uint32_t dilate_cube(uint32_t s) {
+    constexpr uint32_t kCol0 = 0x1249249u;  // dx = -1
+    constexpr uint32_t kCol2 = kCol0 << 2;  // dx = +1
+    constexpr uint32_t kRow0 = 0x01C0E07u;  // dy = -1
+    constexpr uint32_t kRow2 = kRow0 << 6;  // dy = +1
+    constexpr uint32_t kCube = 0x7FFFFFFu;
+    s = (s | ((s << 1) & ~kCol0) | ((s >> 1) & ~kCol2)) & kCube;
+    s = (s | ((s << 3) & ~kRow0) | ((s >> 3) & ~kRow2)) & kCube;
+    return (s | (s << 9) | (s >> 9)) & kCube;
+}
+
+// Whether the ring positions set in `present` form one piece. In a volume
+// the ring's 26 bits go to the cube's bits around the centre (bit 13) and
+// the piece holding the lowest one is grown until it stops: the same
+// closure as a walk over the ring's adjacency lists, held in registers.
 __device__ __forceinline__ bool one_piece(uint32_t present, bool d3, const DeviceTables *t) {
     if (present == 0u || (present & (present - 1u)) == 0u) return true;
     if (!d3) return (t->piece2d[present >> 5] >> (present & 31u)) & 1u;
-    uint32_t reached = present & (~present + 1u);
+    const uint32_t cube = (present & 0x1FFFu) | ((present & ~0x1FFFu) << 1);
+    uint32_t reached = cube & (~cube + 1u);
     for (int round = 0; round < 26; ++round) {
-        uint32_t grown = reached;
-        uint32_t rest = reached;
-        while (rest) {
-            int k = __builtin_ctz(rest);
-            grown |= t->adj3d[k];
-            rest &= rest - 1u;
-        }
-        grown &= present;
+        const uint32_t grown = dilate_cube(reached) & cube;
         if (grown == reached) break;
         reached = grown;
     }
-    return reached == present;
+    return reached == cube;
 }
 
 }  // namespace
@@ -132,22 +144,40 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &a, const int blk) {
         }
     }
 
-    // ---- stage labels (and activity) with whole-row loads.
+    // ---- stage labels (and activity): the tile with its halo as one flat
+    // range over all lanes, four loads in flight per lane before the LDS
+    // stores wait on them (a row at a time leaves each wave waiting on one
+    // load per row).
     const uint32_t *src_lab = (FULL && (a.flags & kMotility)) ? a.labels_prev : a.labels;
-    const int rows = LY * LZ;
-    const int wave = tid >> 6, lane = tid & 63, nwaves = nthreads >> 6;
-    for (int r = wave; r < rows; r += nwaves) {
-        int ly = r % LY, lz = r / LY;
-        int gy = wrap_coord(y0 - halo + ly, a.h);
-        int gz = D3 ? wrap_coord(z0 - halo + lz, a.d) : 0;
-        size_t row = ((size_t)gz * a.h + gy) * (size_t)a.w;
-        uint32_t *dst = sLab + r * LX;
-        for (int c = lane; c < LX; c += 64) {
-            int gx = wrap_coord(x0 - halo + c, a.w);
-            dst[c] = src_lab[row + gx];
-            if (FULL && (a.flags & kMotility)) sAct[r * LX + c] = a.act_prev[row + gx];
+    const bool with_act = FULL && (a.flags & kMotility);
+    const int staged = LX * LY * LZ;
+    for (int base = 0; base < staged; base += 4 * nthreads) {
+        uint32_t lv[4];
+        float av[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int e = base + u * nthreads + tid;
+            if (e < staged) {
+                const int r = e / LX, c = e - r * LX;
+                const int ly = D3 ? r % LY : r, lz = D3 ? r / LY : 0;
+                const int gxx = wrap_coord(x0 - halo + c, a.w);
+                const int gyy = wrap_coord(y0 - halo + ly, a.h);
+                const int gzz = D3 ? wrap_coord(z0 - halo + lz, a.d) : 0;
+                const size_t g = ((size_t)gzz * a.h + gyy) * (size_t)a.w + gxx;
+                lv[u] = src_lab[g];
+                if (with_act) av[u] = a.act_prev[g];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int e = base + u * nthreads + tid;
+            if (e < staged) {
+                sLab[e] = lv[u];
+                if (with_act) sAct[e] = av[u];
+            }
         }
     }
+    const int wave = tid >> 6, lane = tid & 63, nwaves = nthreads >> 6;
     __syncthreads();
 
     // ---- this lane's target.
