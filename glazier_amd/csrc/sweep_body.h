@@ -379,32 +379,41 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &a, const int blk) {
             }
             if (FULL && (a.flags & kMotility) && nid != 0 && tab->coef_act[etn] != 0.0f &&
                 tab->max_act[etn] > 0.0f) {
-                float gm[2];
-                for (int side = 0; side < 2; ++side) {
-                    const int at = side == 0 ? sl : tl;
-                    const uint32_t lab = side == 0 ? nw : old;
-                    float here = sAct[at];
-                    float res = 0.0f;
-                    if (here > 0.0f) {
-                        float lsum = log2_f32(here);
-                        float count = 1.0f;
-                        bool zero = false;
-                        for (int q = 0; q < a.n_off; ++q) {
-                            int nb = at + sOff[q];
-                            if (sLab[nb] != lab) continue;
-                            float v = sAct[nb];
-                            if (!(v > 0.0f)) {
-                                zero = true;
-                                break;
-                            }
-                            lsum = lsum + log2_f32(v);
-                            count = count + 1.0f;
+                // geometric mean of the activity over the source's label
+                // around the source (side 0) and the target's around the
+                // target (side 1); a non-positive value zeroes a side. Both
+                // sides walk the neighbourhood together, so each step's four
+                // LDS reads are in flight at once; the logs add in neighbour
+                // order, and a zeroed side's sum is never read.
+                const float here0 = sAct[sl], here1 = sAct[tl];
+                bool zero0 = !(here0 > 0.0f), zero1 = !(here1 > 0.0f);
+                float lsum0 = zero0 ? 0.0f : log2_f32(here0);
+                float lsum1 = zero1 ? 0.0f : log2_f32(here1);
+                float count0 = 1.0f, count1 = 1.0f;
+                for (int q = 0; q < a.n_off; ++q) {
+                    const int o = sOff[q];
+                    const uint32_t l0 = sLab[sl + o], l1 = sLab[tl + o];
+                    const float v0 = sAct[sl + o], v1 = sAct[tl + o];
+                    if (l0 == nw) {
+                        if (v0 > 0.0f) {
+                            lsum0 = lsum0 + log2_f32(v0);
+                            count0 = count0 + 1.0f;
+                        } else {
+                            zero0 = true;
                         }
-                        res = zero ? 0.0f : exp2_f32(lsum / count);
                     }
-                    gm[side] = res;
+                    if (l1 == old) {
+                        if (v1 > 0.0f) {
+                            lsum1 = lsum1 + log2_f32(v1);
+                            count1 = count1 + 1.0f;
+                        } else {
+                            zero1 = true;
+                        }
+                    }
                 }
-                dh = dh - tab->coef_act[etn] * (gm[0] - gm[1]);
+                const float gm0 = zero0 ? 0.0f : exp2_f32(lsum0 / count0);
+                const float gm1 = zero1 ? 0.0f : exp2_f32(lsum1 / count1);
+                dh = dh - tab->coef_act[etn] * (gm0 - gm1);
             }
             if (FULL && (a.flags & kDrift) && nid != 0) {
                 float wk = tab->ext[etn][0] * (float)(-pdx) + tab->ext[etn][1] * (float)(-pdy) +
