@@ -12,8 +12,10 @@ __global__ void fill_u32_kernel(uint32_t *p, size_t n, uint32_t v) {
     fill_u32_body(p, n, v, blockIdx.x, gridDim.x);
 }
 
-__global__ void consolidate_kernel(int64_t *a, int64_t *e, const uint32_t *n_labels, int cw) {
+__global__ void consolidate_kernel(int64_t *a, int64_t *e, const uint32_t *n_labels, int cw,
+                                   uint32_t *clear4) {
     consolidate_body(a, e, n_labels, cw, blockIdx.x, gridDim.x);
+    if (clear4 && blockIdx.x == 0 && threadIdx.x < 4) clear4[threadIdx.x] = 0u;
 }
 
 __global__ void first_site_kernel(CellArgs a, size_t n_sites) {
@@ -42,6 +44,9 @@ __global__ void check_recentre_kernel(CellArgs a) {
 }
 
 __global__ void plan_kernel(CellArgs a) {
+    // the moved-site counts the next stage accumulates start at zero
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < a.capacity) a.scratch_u32[i] = 0u;
     plan_body(a, blockIdx.x, gridDim.x);
 }
 
@@ -81,6 +86,15 @@ __global__ void copy_u32_kernel(uint32_t *dst, const uint32_t *src, const uint32
     copy_u32_body(dst, src, n_labels, per, blockIdx.x, gridDim.x);
 }
 
+// The compaction's copies of the label types (one word a label) and anchors
+// (three) in one launch.
+__global__ void copy_types_anchors_kernel(uint32_t *types_dst, const uint32_t *types_src,
+                                          uint32_t *anchors_dst, const uint32_t *anchors_src,
+                                          const uint32_t *n_labels) {
+    copy_u32_body(types_dst, types_src, n_labels, 1, blockIdx.x, gridDim.x);
+    copy_u32_body(anchors_dst, anchors_src, n_labels, 3, blockIdx.x, gridDim.x);
+}
+
 __global__ void length_prepass_kernel(const int64_t *a, const int64_t *e, const uint32_t *n_labels, int cw,
                                       const uint32_t *ltype, const DeviceTables *tables, int d3, double *out) {
     length_prepass_body(a, e, n_labels, cw, ltype, tables, d3 != 0, out, blockIdx.x, gridDim.x);
@@ -111,10 +125,10 @@ hipError_t launch_fill_u32(uint32_t *p, size_t n, uint32_t v, hipStream_t s) {
 }
 
 hipError_t launch_consolidate(int64_t *a, int64_t *e, const uint32_t *n_labels, int cw, size_t cap,
-                              hipStream_t s) {
+                              hipStream_t s, uint32_t *clear4) {
     int blocks = blocks_for(cap * cw, 256);
     if (blocks > 4096) blocks = 4096;
-    hipLaunchKernelGGL(consolidate_kernel, dim3(blocks), dim3(256), 0, s, a, e, n_labels, cw);
+    hipLaunchKernelGGL(consolidate_kernel, dim3(blocks), dim3(256), 0, s, a, e, n_labels, cw, clear4);
     return hipGetLastError();
 }
 
@@ -173,8 +187,6 @@ hipError_t launch_population(const CellArgs &a, size_t n_sites, uint32_t *type_c
     const int lb = blocks_for(a.capacity, 256);
     const int sb = blocks_for(n_sites, 256);
     hipLaunchKernelGGL(plan_kernel, dim3(lb), dim3(256), 0, s, a);
-    hipError_t e = launch_fill_u32(a.scratch_u32, a.capacity, 0u, s);
-    if (e != hipSuccess) return e;
     hipLaunchKernelGGL(count_moved_kernel, dim3(sb), dim3(256), 0, s, a, n_sites);
     hipLaunchKernelGGL(assign_daughters_kernel, dim3(1), dim3(1024), 0, s, a);
     hipLaunchKernelGGL(apply_population_kernel, dim3(sb), dim3(256), 0, s, a, n_sites);
@@ -184,10 +196,9 @@ hipError_t launch_population(const CellArgs &a, size_t n_sites, uint32_t *type_c
     {
         int cb = blocks_for(a.capacity, 256);
         if (cb > 4096) cb = 4096;
-        hipLaunchKernelGGL(copy_u32_kernel, dim3(cb), dim3(256), 0, s, type_copy, a.ltype, a.n_labels, 1);
-        hipLaunchKernelGGL(copy_u32_kernel, dim3(cb), dim3(256), 0, s,
+        hipLaunchKernelGGL(copy_types_anchors_kernel, dim3(cb), dim3(256), 0, s, type_copy, a.ltype,
                            reinterpret_cast<uint32_t *>(anchor_copy),
-                           reinterpret_cast<const uint32_t *>(a.anchor), a.n_labels, 3);
+                           reinterpret_cast<const uint32_t *>(a.anchor), a.n_labels);
     }
     hipLaunchKernelGGL(compact_move_kernel, dim3(lb), dim3(256), 0, s, a, type_copy, anchor_copy);
     hipLaunchKernelGGL(compact_relabel_kernel, dim3(sb), dim3(256), 0, s, a, n_sites);

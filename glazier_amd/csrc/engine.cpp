@@ -400,11 +400,11 @@ ghip_status run_fields(ghip_sim *s) {
 ghip_status run_cells(ghip_sim *s) {
     // bring the per-label state current: A[p%2] += D[(p+2)%3]
     const uint64_t p = s->pass;
+    // (and clear the step's flags)
     HIP_TRY(launch_consolidate(s->A[p % 2], s->Dl[(p + 2) % 3], s->n_labels_dev, s->cw, s->cap,
-                               s->stream));
+                               s->stream, s->flags));
     TimedLaunch t{};
     HIP_TRY(s->begin_timed(3, &t));
-    HIP_TRY(launch_fill_u32(s->flags, 4, 0u, s->stream));
     CellArgs c = s->cell_args();
     if (s->has_pop) {
         c.key = pass_key(s->m.seed, s->m.replica, s->mcs, 0xDEADCE11U);

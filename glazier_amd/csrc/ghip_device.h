@@ -214,8 +214,10 @@ hipError_t launch_length_prepass(const int64_t *a, const int64_t *e, const uint3
                                  uint32_t capacity, hipStream_t stream);
 hipError_t launch_fill_i64(int64_t *p, size_t n, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t *p, size_t n, uint32_t v, hipStream_t s);
+// A[i] += E[i], E[i] = 0 over the live labels' words; with `clear4` it also
+// zeroes those four flag words.
 hipError_t launch_consolidate(int64_t *a, int64_t *e, const uint32_t *n_labels, int cw, size_t cap,
-                              hipStream_t s);
+                              hipStream_t s, uint32_t *clear4 = nullptr);
 hipError_t launch_recount(const CellArgs &a, size_t n_sites, hipStream_t s);
 // Recount of the moment words (1..cw-1, nematic sums included) about the
 // current anchors, volume and surface kept.
