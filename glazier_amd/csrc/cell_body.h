@@ -267,13 +267,33 @@ __device__ __forceinline__ void count_moved_body(const CellArgs &a, size_t n_sit
 // One workgroup: cancel divisions that would move nothing or everything,
 // hand out daughter ids in ascending parent order, and copy types and
 // anchors to the daughters.
+// Rank of this lane's flag among the workgroup's set flags (lanes in
+// order), and their count: a ballot per wavefront and one barrier-bounded
+// pass over the wavefronts' counts, instead of a scan over every lane. The
+// workgroup is a whole number of wavefronts, at most 16.
+__device__ __forceinline__ uint32_t block_rank(bool flag, uint32_t *wave_counts, uint32_t *total) {
+    const unsigned lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const unsigned long long m = __ballot(flag);
+    const uint32_t below = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+    if (lane == 0) wave_counts[wave] = (uint32_t)__popcll(m);
+    __syncthreads();
+    uint32_t before = 0, all = 0;
+    const unsigned waves = blockDim.x >> 6;
+    for (unsigned w = 0; w < waves; ++w) {
+        const uint32_t c = wave_counts[w];
+        before += w < wave ? c : 0u;
+        all += c;
+    }
+    __syncthreads();
+    *total = all;
+    return before + below;
+}
+
 __device__ __forceinline__ void assign_daughters_body(const CellArgs &a, const int lb, const int gb) {
     if (a.flags[0] == 0) return;
-    __shared__ uint32_t part[1024];
-    __shared__ uint32_t base;
+    __shared__ uint32_t wave_counts[16];
     const uint32_t n0 = *a.n_labels;
-    if (threadIdx.x == 0) base = n0;
-    __syncthreads();
+    uint32_t base = n0;
     for (uint32_t start = 0; start < n0; start += blockDim.x) {
         uint32_t l = start + threadIdx.x;
         uint32_t take = 0;
@@ -285,16 +305,10 @@ __device__ __forceinline__ void assign_daughters_body(const CellArgs &a, const i
             else
                 take = 1;
         }
-        part[threadIdx.x] = take;
-        __syncthreads();
-        for (int off = 1; off < (int)blockDim.x; off <<= 1) {
-            uint32_t add = threadIdx.x >= (unsigned)off ? part[threadIdx.x - off] : 0;
-            __syncthreads();
-            part[threadIdx.x] += add;
-            __syncthreads();
-        }
+        uint32_t taken = 0;
+        const uint32_t rank = block_rank(take != 0, wave_counts, &taken);
         if (take) {
-            uint32_t id = base + part[threadIdx.x] - 1;
+            uint32_t id = base + rank;
             if (id >= a.capacity) {
                 a.plan[l] = 0;
                 a.flags[2] = 1;
@@ -306,9 +320,7 @@ __device__ __forceinline__ void assign_daughters_body(const CellArgs &a, const i
                 a.anchor[3 * id + 2] = a.anchor[3 * l + 2];
             }
         }
-        __syncthreads();
-        if (threadIdx.x == blockDim.x - 1) base += part[threadIdx.x];
-        __syncthreads();
+        base += taken;
     }
     if (threadIdx.x == 0) *a.n_labels = base < a.capacity ? base : a.capacity;
 }
@@ -349,14 +361,9 @@ __device__ __forceinline__ void zero_state_body(const CellArgs &a, int gate, con
 // flags[3].
 __device__ __forceinline__ void compact_plan_body(const CellArgs &a, const int lb, const int gb) {
     if (a.flags[0] == 0) return;
-    __shared__ uint32_t part[1024];
     __shared__ uint32_t dead_total;
-    __shared__ uint32_t base;
     const uint32_t n = *a.n_labels;
-    if (threadIdx.x == 0) {
-        dead_total = 0;
-        base = 1;
-    }
+    if (threadIdx.x == 0) dead_total = 0;
     __syncthreads();
     for (uint32_t l = 1 + threadIdx.x; l < n; l += blockDim.x)
         if (unpack_vol(a.state[(size_t)l * a.cw]) <= 0) atomicAdd(&dead_total, 1u);
@@ -364,25 +371,19 @@ __device__ __forceinline__ void compact_plan_body(const CellArgs &a, const int l
     uint32_t limit = (n - 1) / 4;
     if (limit < 64) limit = 64;
     if (dead_total <= limit) return;
+    __shared__ uint32_t wave_counts[16];
+    uint32_t next = 1;
     for (uint32_t start = 1; start < n; start += blockDim.x) {
         uint32_t l = start + threadIdx.x;
         uint32_t live = (l < n && unpack_vol(a.state[(size_t)l * a.cw]) > 0) ? 1u : 0u;
-        part[threadIdx.x] = live;
-        __syncthreads();
-        for (int off = 1; off < (int)blockDim.x; off <<= 1) {
-            uint32_t add = threadIdx.x >= (unsigned)off ? part[threadIdx.x - off] : 0;
-            __syncthreads();
-            part[threadIdx.x] += add;
-            __syncthreads();
-        }
-        if (l < n) a.scratch_u32[l] = live ? base + part[threadIdx.x] - 1 : 0u;
-        __syncthreads();
-        if (threadIdx.x == blockDim.x - 1) base += part[threadIdx.x];
-        __syncthreads();
+        uint32_t lived = 0;
+        const uint32_t rank = block_rank(live != 0, wave_counts, &lived);
+        if (l < n) a.scratch_u32[l] = live ? next + rank : 0u;
+        next += lived;
     }
     if (threadIdx.x == 0) {
         a.scratch_u32[0] = 0;
-        a.flags[3] = base; // the new label count, nonzero
+        a.flags[3] = next; // the new label count, nonzero
     }
 }
 
