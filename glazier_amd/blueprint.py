@@ -146,8 +146,6 @@ class Blueprint:
                        depth=int(raw.get("depth", 1)), fields=species,
                        adhesion=raw.get("adhesion"), nematic_field=raw.get("nematic_field"),
                        base=Path(base))
-        bp.initial.setdefault("nz", 1)
-        bp.initial.setdefault("fractions", {})
         bp._check()
         return bp
 
@@ -159,7 +157,7 @@ class Blueprint:
     def _check(self) -> None:
         """The checks of Blueprint::from_json (blueprint.rs:386-446)."""
         names = [t.name for t in self.types]
-        for name in self.initial["fractions"]:
+        for name in self.initial.get("fractions", {}):
             if name not in names:
                 raise ValueError(f"the initial condition names the type {name}, which the "
                                  "description does not state")
@@ -209,7 +207,7 @@ class Blueprint:
     def initial_types(self, cells: int) -> List[int]:
         out = [1] * cells
         nxt = 0
-        fractions = self.initial["fractions"]
+        fractions = self.initial.get("fractions", {})
         for index, spec in enumerate(self.types):
             if spec.name not in fractions:
                 continue

@@ -98,3 +98,57 @@ def test_the_serial_oracle_reproduces_the_published_cpu_column():
     assert f"{row['mean_eccentricity']:.3f}" == "0.397"
     assert f"{row['nematic_order']:.3f}" == "0.064"
     assert row["defects"] == 56 and row["net_charge"] == 0.0
+
+
+# ---- pinned to the reference's measurements of fixed label fields
+# (tests/golden/director_vectors.npz, written by the reference's
+# python/glazier/director.py through tests/golden/make_reference_vectors.py)
+
+def _vectors():
+    from pathlib import Path
+
+    path = Path(__file__).resolve().parent / "golden" / "director_vectors.npz"
+    with np.load(path, allow_pickle=False) as z:
+        data = {k: z[k] for k in z.files}
+    names = sorted({k.split("__")[0] for k in data})
+    return names, data
+
+
+def _angle_gap(a, b):
+    """Distance of two angles taken mod pi."""
+    d = np.mod(np.asarray(a) - np.asarray(b) + 0.5 * np.pi, np.pi) - 0.5 * np.pi
+    return np.abs(d)
+
+
+def test_measurements_match_the_reference_on_fixed_fields():
+    names, v = _vectors()
+    assert names == ["brick", "ragged", "volume"]
+    for name in names:
+        labels = v[f"{name}__labels"]
+        plane = director.mid_plane(labels)
+        assert np.array_equal(plane, v[f"{name}__mid_plane"]), name
+        shapes = director.cell_shapes(labels)
+        want = v[f"{name}__shapes"]
+        got = np.array([[s.label, s.area, s.centre[0], s.centre[1], s.theta, s.eccentricity]
+                        for s in shapes])
+        assert got.shape == want.shape, name
+        assert np.array_equal(got[:, :2], want[:, :2]), name  # labels and areas exactly
+        assert np.allclose(got[:, 2:4], want[:, 2:4], rtol=0, atol=1e-9), name
+        assert _angle_gap(got[:, 4], want[:, 4]).max() < 1e-9, name
+        assert np.allclose(got[:, 5], want[:, 5], rtol=0, atol=1e-9), name
+        q = director.q_field(labels, shapes)
+        assert np.allclose(q, v[f"{name}__q"], rtol=0, atol=1e-9), name
+        qc = director.coarse_grain(q, 4.0)
+        assert np.allclose(qc, v[f"{name}__q_coarse"], rtol=0, atol=1e-9), name
+        theta = director.director_angle(qc)
+        assert _angle_gap(theta, v[f"{name}__theta"]).max() < 1e-6, name
+        order = 2.0 * np.hypot(qc[..., 0], qc[..., 1])
+        assert np.allclose(order, v[f"{name}__order"], rtol=0, atol=1e-9), name
+        glob = director.global_order(shapes)
+        assert abs(glob[0] - v[f"{name}__global_order"][0]) < 1e-9, name
+        assert _angle_gap(glob[1], v[f"{name}__global_order"][1]) < 1e-9, name
+        got_defects = np.array(director.defect_charges(theta), dtype=np.float64).reshape(-1, 3)
+        want_defects = v[f"{name}__defects"]
+        assert got_defects.shape == want_defects.shape, name
+        assert np.array_equal(got_defects[:, :2], want_defects[:, :2]), name
+        assert np.allclose(got_defects[:, 2], want_defects[:, 2], rtol=0, atol=1e-9), name
