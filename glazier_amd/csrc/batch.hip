@@ -194,7 +194,8 @@ __global__ __launch_bounds__(THREADS) void batch_cell_kernel(BatchArgs b, WorkLi
         const uint64_t p = passes_before(R, b.mcs[r] + 1);
         consolidate_body(R.A[p % 2], R.D[(p + 2) % 3], c.n_labels, c.cw, lb, gb);
     } else if constexpr (OP == kOpFlags) {
-        fill_u32_body(c.flags, 4, 0u, lb, gb);
+        // word 2 (ids ran out) is kept for the host to read after the call
+        if (lb == 0 && threadIdx.x < 4 && threadIdx.x != 2) c.flags[threadIdx.x] = 0u;
     } else if constexpr (OP == kOpPlan) {
         plan_body(c, lb, gb);
     } else if constexpr (OP == kOpFillScratch) {
@@ -212,8 +213,9 @@ __global__ __launch_bounds__(THREADS) void batch_cell_kernel(BatchArgs b, WorkLi
     } else if constexpr (OP == kOpCompactPlan) {
         compact_plan_body(c, lb, gb);
     } else if constexpr (OP == kOpCopyTypes) {
-        copy_u32_body(R.type_copy, c.ltype, c.n_labels, 1, lb, gb);
+        if (c.flags[0] != 0 && c.flags[3] != 0) copy_u32_body(R.type_copy, c.ltype, c.n_labels, 1, lb, gb);
     } else if constexpr (OP == kOpCopyAnchors) {
+        if (c.flags[0] == 0 || c.flags[3] == 0) return;
         copy_u32_body(reinterpret_cast<uint32_t *>(R.anchor_copy),
                       reinterpret_cast<const uint32_t *>(c.anchor), c.n_labels, 3, lb, gb);
     } else if constexpr (OP == kOpCompactMove) {

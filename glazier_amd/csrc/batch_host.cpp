@@ -436,8 +436,11 @@ ghip_status check_capacity(ghip_batch *b) {
         if (!s->has_pop) continue;
         uint32_t fl[4];
         HIP_TRY(hipMemcpy(fl, s->flags, 16, hipMemcpyDeviceToHost));
-        if (fl[2]) return fail(GHIP_DEVICE, "the label capacity of a lattice in the batch ran out; "
-                                            "divisions were dropped");
+        if (fl[2]) {
+            HIP_TRY(hipMemset(s->flags + 2, 0, 4));
+            return fail(GHIP_DEVICE, "a lattice in the batch reached the label id limit; "
+                                     "divisions were dropped");
+        }
     }
     return GHIP_OK;
 }
@@ -549,20 +552,12 @@ ghip_status ghip_batch_set_tiles(ghip_batch *b, int candidate) {
     return rebuild(b);
 }
 
-ghip_status ghip_batch_step(ghip_batch *b, uint64_t steps) {
-    if (!b) return fail(GHIP_BAD_ARG, "null handle");
-    for (ghip_sim *s : b->sims)
-        if (s->traj) return fail(GHIP_UNSUPPORTED, "a lattice in the batch is recording its trajectory; "
-                                                   "step it alone");
-    bool stale = false;
-    for (size_t r = 0; r < b->sims.size(); ++r) {
-        HIP_TRY(hipStreamSynchronize(b->sims[r]->stream));
-        if (b->sims[r]->mcs != b->expect_mcs[r]) stale = true;
-    }
-    if (stale) {
-        ghip_status st = rebuild(b);
-        if (st != GHIP_OK) return st;
-    }
+}  // extern "C"
+
+namespace {
+
+// `steps` steps of every lattice, none of which can run out of label ids.
+ghip_status batch_run(ghip_batch *b, uint64_t steps) {
     uint64_t q = 0;
     // tuning: real steps, one candidate shape each, timed per group
     while (q < steps && b->autotune && !b->tuned) {
@@ -622,6 +617,48 @@ ghip_status ghip_batch_step(ghip_batch *b, uint64_t steps) {
         HIP_TRY(hipStreamSynchronize(b->stream));
         advance_host(b, steps - q);
         b->graph_steps += steps - q;
+    }
+    return GHIP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+ghip_status ghip_batch_step(ghip_batch *b, uint64_t steps) {
+    if (!b) return fail(GHIP_BAD_ARG, "null handle");
+    for (ghip_sim *s : b->sims)
+        if (s->traj) return fail(GHIP_UNSUPPORTED, "a lattice in the batch is recording its trajectory; "
+                                                   "step it alone");
+    bool stale = false;
+    for (size_t r = 0; r < b->sims.size(); ++r) {
+        HIP_TRY(hipStreamSynchronize(b->sims[r]->stream));
+        if (b->sims[r]->mcs != b->expect_mcs[r]) stale = true;
+    }
+    if (stale) {
+        ghip_status st = rebuild(b);
+        if (st != GHIP_OK) return st;
+    }
+    while (steps > 0) {
+        // every lattice that divides gets room for the chunk; one that had
+        // to grow moved its buffers, so the work lists and graph are rebuilt
+        uint64_t take = steps;
+        bool moved = false;
+        for (ghip_sim *s : b->sims) {
+            uint64_t chunk = 0;
+            bool grew = false;
+            ghip_status st = ensure_label_room(s, take, &chunk, &grew);
+            if (st != GHIP_OK) return st;
+            take = std::min(take, chunk);
+            moved = moved || grew;
+        }
+        if (moved) {
+            ghip_status st = rebuild(b);
+            if (st != GHIP_OK) return st;
+        }
+        ghip_status st = batch_run(b, take);
+        if (st != GHIP_OK) return st;
+        steps -= take;
     }
     return check_capacity(b);
 }

@@ -15,7 +15,9 @@ __global__ void fill_u32_kernel(uint32_t *p, size_t n, uint32_t v) {
 __global__ void consolidate_kernel(int64_t *a, int64_t *e, const uint32_t *n_labels, int cw,
                                    uint32_t *clear4) {
     consolidate_body(a, e, n_labels, cw, blockIdx.x, gridDim.x);
-    if (clear4 && blockIdx.x == 0 && threadIdx.x < 4) clear4[threadIdx.x] = 0u;
+    // the step's flags start clear, except word 2 (ids ran out), which the
+    // host reads and resets once the call is done
+    if (clear4 && blockIdx.x == 0 && threadIdx.x < 4 && threadIdx.x != 2) clear4[threadIdx.x] = 0u;
 }
 
 __global__ void first_site_kernel(CellArgs a, size_t n_sites) {
@@ -87,10 +89,11 @@ __global__ void copy_u32_kernel(uint32_t *dst, const uint32_t *src, const uint32
 }
 
 // The compaction's copies of the label types (one word a label) and anchors
-// (three) in one launch.
+// (three) in one launch, only on a step whose plan compacts.
 __global__ void copy_types_anchors_kernel(uint32_t *types_dst, const uint32_t *types_src,
                                           uint32_t *anchors_dst, const uint32_t *anchors_src,
-                                          const uint32_t *n_labels) {
+                                          const uint32_t *n_labels, const uint32_t *flags) {
+    if (flags[0] == 0 || flags[3] == 0) return;
     copy_u32_body(types_dst, types_src, n_labels, 1, blockIdx.x, gridDim.x);
     copy_u32_body(anchors_dst, anchors_src, n_labels, 3, blockIdx.x, gridDim.x);
 }
@@ -198,7 +201,7 @@ hipError_t launch_population(const CellArgs &a, size_t n_sites, uint32_t *type_c
         if (cb > 4096) cb = 4096;
         hipLaunchKernelGGL(copy_types_anchors_kernel, dim3(cb), dim3(256), 0, s, type_copy, a.ltype,
                            reinterpret_cast<uint32_t *>(anchor_copy),
-                           reinterpret_cast<const uint32_t *>(a.anchor), a.n_labels);
+                           reinterpret_cast<const uint32_t *>(a.anchor), a.n_labels, a.flags);
     }
     hipLaunchKernelGGL(compact_move_kernel, dim3(lb), dim3(256), 0, s, a, type_copy, anchor_copy);
     hipLaunchKernelGGL(compact_relabel_kernel, dim3(sb), dim3(256), 0, s, a, n_sites);

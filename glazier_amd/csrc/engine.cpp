@@ -416,6 +416,86 @@ ghip_status run_cells(ghip_sim *s) {
     return GHIP_OK;
 }
 
+// Labels one step can add at most: every dividing cell holds at least
+// max(4, division volume) sites and divides once, so a step adds at most
+// N / that many labels. Zero when no type divides.
+uint64_t labels_per_step_bound(const ghip_sim *s) {
+    double dv = 0.0;
+    for (uint32_t t = 1; t < s->m.n_types; ++t) {
+        const double v = s->m.types[t].division_volume;
+        if (v > 0.0 && (dv == 0.0 || v < dv)) dv = v;
+    }
+    if (dv == 0.0) return 0;
+    const double least = std::max(4.0, std::ceil(dv));
+    return (uint64_t)((double)s->N / least);
+}
+
+// Copies `old_bytes` of *p into a new zeroed buffer of `new_bytes`.
+hipError_t regrow(void **p, size_t old_bytes, size_t new_bytes) {
+    if (!*p) return hipSuccess;
+    void *q = nullptr;
+    hipError_t e = hipMalloc(&q, new_bytes);
+    if (e != hipSuccess) return e;
+    e = hipMemset(q, 0, new_bytes);
+    if (e == hipSuccess && old_bytes) e = hipMemcpy(q, *p, old_bytes, hipMemcpyDeviceToDevice);
+    if (e != hipSuccess) {
+        (void)hipFree(q);
+        return e;
+    }
+    (void)hipFree(*p);
+    *p = q;
+    return hipSuccess;
+}
+
+}  // namespace
+
+namespace ghip {
+
+// Makes room for the divisions of the next steps, as the reference grows its
+// per-label buffers when daughters outnumber them (glazier-cuda/src/lib.rs:
+// 1446-1459). With the stream idle, reads the label count and, when fewer
+// than one step's worth of ids is free, moves every per-label buffer to a
+// larger capacity (contents kept, the rest zeroed). *chunk receives how many
+// of `steps` can run before the next check with no chance of running out.
+ghip_status ensure_label_room(ghip_sim *s, uint64_t steps, uint64_t *chunk, bool *grew) {
+    *chunk = steps;
+    if (grew) *grew = false;
+    const uint64_t per_step = labels_per_step_bound(s);
+    if (!s->has_pop || per_step == 0 || steps == 0) return GHIP_OK;
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    uint32_t n = 0;
+    HIP_TRY(hipMemcpy(&n, s->n_labels_dev, 4, hipMemcpyDeviceToHost));
+    const uint64_t limit = kIdMask - 1;
+    if ((uint64_t)s->cap - n < per_step && s->cap < limit) {
+        uint64_t want = std::max<uint64_t>(2ull * s->cap, (uint64_t)n + 8 * per_step);
+        want = std::min(want, limit);
+        const size_t oc = s->cap, nc = (size_t)want, cw = (size_t)s->cw;
+        for (int i = 0; i < 2; ++i)
+            HIP_TRY(regrow((void **)&s->A[i], oc * cw * 8, nc * cw * 8));
+        for (int i = 0; i < 3; ++i)
+            HIP_TRY(regrow((void **)&s->Dl[i], oc * cw * 8, nc * cw * 8));
+        HIP_TRY(regrow((void **)&s->anchor, oc * 12, nc * 12));
+        HIP_TRY(regrow((void **)&s->ltype, oc * 4, nc * 4));
+        HIP_TRY(regrow((void **)&s->scratch, oc * 4, nc * 4));
+        HIP_TRY(regrow((void **)&s->daughter, oc * 4, nc * 4));
+        HIP_TRY(regrow((void **)&s->plan, oc, nc));
+        HIP_TRY(regrow((void **)&s->geo, oc * 48, nc * 48));
+        HIP_TRY(regrow((void **)&s->len_before, oc * 8, nc * 8));
+        HIP_TRY(regrow((void **)&s->type_copy, oc * 4, nc * 4));
+        HIP_TRY(regrow((void **)&s->anchor_copy, oc * 12, nc * 12));
+        s->cap = (uint32_t)nc;
+        s->grown += 1;
+        if (grew) *grew = true;
+    }
+    const uint64_t room = s->cap > n ? (uint64_t)s->cap - n : 0;
+    *chunk = std::max<uint64_t>(1, std::min(steps, room / per_step));
+    return GHIP_OK;
+}
+
+}  // namespace ghip
+
+namespace {
+
 ghip_status read_state(const ghip_sim *s, std::vector<int64_t> &cur, uint32_t &n_labels) {
     HIP_TRY(hipStreamSynchronize(s->stream));
     HIP_TRY(hipMemcpy(&n_labels, s->n_labels_dev, 4, hipMemcpyDeviceToHost));
@@ -674,7 +754,13 @@ ghip_status ghip_step(ghip_sim *s, uint64_t steps) {
         if (st != GHIP_OK) return st;
         steps = 0;
     }
+    uint64_t chunk = 0;
     for (uint64_t q = 0; q < steps; ++q) {
+        if (chunk == 0) {
+            ghip_status st = ensure_label_room(s, steps - q, &chunk, nullptr);
+            if (st != GHIP_OK) return st;
+        }
+        --chunk;
         int order[8];
         colour_order(s->m.seed, s->m.replica, s->mcs, s->colours, order);
         for (int c = 0; c < s->colours; ++c) {
@@ -708,9 +794,13 @@ ghip_status ghip_step(ghip_sim *s, uint64_t steps) {
         s->launches[kStepFamily] += s->pass - pass0;
     }
     if (s->has_pop) {
+        // word 2 outlives the per-step clears: set by any step of the call
         uint32_t fl[4];
         HIP_TRY(hipMemcpy(fl, s->flags, 16, hipMemcpyDeviceToHost));
-        if (fl[2]) return fail(GHIP_DEVICE, "the label capacity ran out; divisions were dropped");
+        if (fl[2]) {
+            HIP_TRY(hipMemset(s->flags + 2, 0, 4));
+            return fail(GHIP_DEVICE, "the label capacity ran out at the id limit; divisions were dropped");
+        }
     }
     return GHIP_OK;
 }

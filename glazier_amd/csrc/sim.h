@@ -108,6 +108,7 @@ struct ghip_sim {
     uint64_t launches[5] = {0, 0, 0, 0, 0};
 
     ghip_batch *batch = nullptr;  // the batch this lattice is stepped in, if any
+    uint32_t grown = 0;           // times the per-label buffers moved to a larger capacity
 
     // owner-type trajectory: one byte per site per recorded step
     uint8_t *traj = nullptr;
@@ -225,3 +226,10 @@ struct ghip_sim {
         return hipSuccess;
     }
 };
+
+namespace ghip {
+// Room for the divisions of the next `steps` (engine.cpp): grows the
+// per-label buffers when needed and gives how many steps may run before the
+// next check.
+ghip_status ensure_label_room(ghip_sim *s, uint64_t steps, uint64_t *chunk, bool *grew);
+}  // namespace ghip

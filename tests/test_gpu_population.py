@@ -92,3 +92,52 @@ def test_both_engines_lose_cells_at_the_rate_the_description_states(engine, orac
     for name, left in (("serial", live(cpu.volumes())), ("device", live(gpu.volumes()))):
         assert abs(left - expected) < 3.0 * spread, (name, left, expected, spread)
     gpu.close()
+
+
+# ---- label capacity: grown as the reference grows its per-label buffers
+# (glazier-cuda/src/lib.rs:1446-1459), so divisions past the first
+# allocation are never dropped and the run stays the emulation's.
+
+def dividing_sheet(seed):
+    # 4x4 cells that want 32 sites and divide at 8: every cell divides in the
+    # first step and about half again in the second, past the 2n + 1024 ids
+    # set aside at creation (n = 4097 labels, 9218 ids)
+    c = CellType(target_volume=32.0, lambda_volume=2.0, division_volume=8.0)
+    m = Model(width=256, height=256, contact=[0.0, 16.0, 16.0, 8.0], types=[CellType(), c],
+              temperature=10.0, neighbour_order=2, seed=seed)
+    labels, types = tiled(m, side=4, nx=64, ny=64)
+    return m, labels, types
+
+
+def test_divisions_past_the_first_allocation_match_the_emulation(engine):
+    from glazier_amd import HipSimulation
+
+    m, labels, types = dividing_sheet(11)
+    gpu = HipSimulation(m, labels, types)
+    emu = oracle.EmuSimulation(m, labels, types)
+    for chunk in (1, 5):  # a call that divides at once, then one that runs on
+        gpu.step(chunk)
+        emu.step(chunk)
+        assert np.array_equal(gpu.labels(), emu.labels())
+        assert np.array_equal(gpu.cell_state()[1:], emu.cell_state()[1:])
+    assert gpu.n_labels() > 2 * 4097 + 1024, gpu.n_labels()
+    gpu.close()
+
+
+def test_a_batch_grows_its_lattices_and_matches_the_emulation(engine):
+    from glazier_amd import HipBatch, HipSimulation
+
+    sims, emus = [], []
+    for seed in (12, 13):
+        m, labels, types = dividing_sheet(seed)
+        sims.append(HipSimulation(m, labels, types))
+        emus.append(oracle.EmuSimulation(m, labels, types))
+    with HipBatch(sims) as batch:
+        batch.set_tiles(0)
+        batch.step(4)
+    for gpu, emu in zip(sims, emus):
+        emu.step(4)
+        assert gpu.n_labels() > 2 * 4097 + 1024
+        assert np.array_equal(gpu.labels(), emu.labels())
+        assert np.array_equal(gpu.cell_state()[1:], emu.cell_state()[1:])
+        gpu.close()
