@@ -68,28 +68,19 @@ struct WorkList {
 };
 
 // The per-label launches of a step, in the order the single-lattice driver
-// enqueues them (consolidation, flags, division/death and compaction, then
-// the recentring check), plus the activity decay.
+// enqueues them (consolidation with the flag clear, division/death whose
+// plan checks for drift, compaction, then the recentring; lattices without
+// division or death check for drift on their own), plus the activity decay.
 enum BatchCellOp : int {
     kOpConsolidate,
-    kOpFlags,
     kOpPlan,
-    kOpFillScratch,
     kOpCountMoved,
     kOpAssign,
     kOpApply,
-    kOpZeroState0,
-    kOpRecount0,
     kOpCompactPlan,
-    kOpCopyTypes,
-    kOpCopyAnchors,
+    kOpCompactStage,
     kOpCompactMove,
     kOpCompactRelabel,
-    kOpCompactFinish,
-    kOpZeroState3,
-    kOpRecount3,
-    kOpRecentre0,
-    kOpRecountMoments0,
     kOpCheckRecentre,
     kOpRecentre1,
     kOpRecountMoments1,

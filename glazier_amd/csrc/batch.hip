@@ -170,6 +170,7 @@ __global__ void batch_prepare_cells_kernel(BatchArgs b) {
     const uint64_t m = b.mcs[r];
     const uint64_t p_end = passes_before(R, m + 1);
     c.state = R.A[p_end % 2];
+    c.state_copy = R.A[(p_end + 1) % 2];
     c.labels = R.labels[flip_index(R, R.lab0, p_end)];
     c.key = pass_key(R.seed, R.replica, m, 0xDEADCE11U);
     c.key2 = second_key(c.key);
@@ -193,45 +194,28 @@ __global__ __launch_bounds__(THREADS) void batch_cell_kernel(BatchArgs b, WorkLi
     if constexpr (OP == kOpConsolidate) {
         const uint64_t p = passes_before(R, b.mcs[r] + 1);
         consolidate_body(R.A[p % 2], R.D[(p + 2) % 3], c.n_labels, c.cw, lb, gb);
-    } else if constexpr (OP == kOpFlags) {
-        // word 2 (ids ran out) is kept for the host to read after the call
+        // the step's flags start clear, except word 2 (ids ran out), which
+        // the host reads and resets after the call
         if (lb == 0 && threadIdx.x < 4 && threadIdx.x != 2) c.flags[threadIdx.x] = 0u;
     } else if constexpr (OP == kOpPlan) {
+        // the moved-site counts the next stage accumulates start at zero
+        const uint32_t i = lb * blockDim.x + threadIdx.x;
+        if (i < c.capacity) c.scratch_u32[i] = 0u;
         plan_body(c, lb, gb);
-    } else if constexpr (OP == kOpFillScratch) {
-        fill_u32_body(c.scratch_u32, c.capacity, 0u, lb, gb);
     } else if constexpr (OP == kOpCountMoved) {
         count_moved_body(c, n, lb, gb);
     } else if constexpr (OP == kOpAssign) {
         assign_daughters_body(c, lb, gb);
     } else if constexpr (OP == kOpApply) {
         apply_population_body(c, n, lb, gb);
-    } else if constexpr (OP == kOpZeroState0) {
-        zero_state_body(c, 0, lb, gb);
-    } else if constexpr (OP == kOpRecount0) {
-        recount_body(c, n, 1024, 0, 0, lb, gb);
     } else if constexpr (OP == kOpCompactPlan) {
         compact_plan_body(c, lb, gb);
-    } else if constexpr (OP == kOpCopyTypes) {
-        if (c.flags[0] != 0 && c.flags[3] != 0) copy_u32_body(R.type_copy, c.ltype, c.n_labels, 1, lb, gb);
-    } else if constexpr (OP == kOpCopyAnchors) {
-        if (c.flags[0] == 0 || c.flags[3] == 0) return;
-        copy_u32_body(reinterpret_cast<uint32_t *>(R.anchor_copy),
-                      reinterpret_cast<const uint32_t *>(c.anchor), c.n_labels, 3, lb, gb);
+    } else if constexpr (OP == kOpCompactStage) {
+        compact_stage_body(c, R.type_copy, R.anchor_copy, lb, gb);
     } else if constexpr (OP == kOpCompactMove) {
         compact_move_body(c, R.type_copy, R.anchor_copy, lb, gb);
     } else if constexpr (OP == kOpCompactRelabel) {
         compact_relabel_body(c, n, lb, gb);
-    } else if constexpr (OP == kOpCompactFinish) {
-        compact_finish_body(c, lb, gb);
-    } else if constexpr (OP == kOpZeroState3) {
-        zero_state_body(c, 3, lb, gb);
-    } else if constexpr (OP == kOpRecount3) {
-        recount_body(c, n, 1024, 0, 3, lb, gb);
-    } else if constexpr (OP == kOpRecentre0) {
-        recentre_body(c, 0, lb, gb);
-    } else if constexpr (OP == kOpRecountMoments0) {
-        recount_body(c, n, 1024, 1, 0, lb, gb);
     } else if constexpr (OP == kOpCheckRecentre) {
         check_recentre_body(c, lb, gb);
     } else if constexpr (OP == kOpRecentre1) {
@@ -320,24 +304,14 @@ hipError_t launch_batch_cell(int op, const BatchArgs &b, const WorkList &w, hipS
         break
     switch (op) {
         GHIP_BC(kOpConsolidate, 256);
-        GHIP_BC(kOpFlags, 256);
         GHIP_BC(kOpPlan, 256);
-        GHIP_BC(kOpFillScratch, 256);
         GHIP_BC(kOpCountMoved, 256);
         GHIP_BC(kOpAssign, 1024);
         GHIP_BC(kOpApply, 256);
-        GHIP_BC(kOpZeroState0, 256);
-        GHIP_BC(kOpRecount0, 256);
         GHIP_BC(kOpCompactPlan, 1024);
-        GHIP_BC(kOpCopyTypes, 256);
-        GHIP_BC(kOpCopyAnchors, 256);
+        GHIP_BC(kOpCompactStage, 256);
         GHIP_BC(kOpCompactMove, 256);
         GHIP_BC(kOpCompactRelabel, 256);
-        GHIP_BC(kOpCompactFinish, 256);
-        GHIP_BC(kOpZeroState3, 256);
-        GHIP_BC(kOpRecount3, 256);
-        GHIP_BC(kOpRecentre0, 256);
-        GHIP_BC(kOpRecountMoments0, 256);
         GHIP_BC(kOpCheckRecentre, 256);
         GHIP_BC(kOpRecentre1, 256);
         GHIP_BC(kOpRecountMoments1, 256);

@@ -57,25 +57,16 @@ uint32_t cell_blocks(const ghip_sim *s, int op) {
     const uint64_t words = (uint64_t)s->cap * s->cw;
     switch (op) {
         case kOpConsolidate:
-        case kOpZeroState0:
-        case kOpZeroState3:
+        case kOpCompactStage:
             return std::min<uint32_t>(4096, cdiv(words, 256));
-        case kOpFlags:
         case kOpAssign:
         case kOpCompactPlan:
-        case kOpCompactFinish:
             return 1;
-        case kOpCopyTypes:
-        case kOpCopyAnchors:
-            return std::min<uint32_t>(4096, cdiv(s->cap, 256));
         case kOpCountMoved:
         case kOpApply:
         case kOpCompactRelabel:
         case kOpDecay:
             return cdiv(s->N, 256);
-        case kOpRecount0:
-        case kOpRecount3:
-        case kOpRecountMoments0:
         case kOpRecountMoments1:
             return cdiv(s->N, 1024);
         default:
@@ -83,7 +74,7 @@ uint32_t cell_blocks(const ghip_sim *s, int op) {
     }
 }
 
-bool op_is_population(int op) { return op >= kOpPlan && op <= kOpRecountMoments0; }
+bool op_is_population(int op) { return op >= kOpPlan && op <= kOpCompactRelabel; }
 
 // Host form of a work list: offsets into the batch's index pool.
 struct ListRef {
@@ -325,9 +316,11 @@ ghip_status rebuild(ghip_batch *b) {
         std::vector<uint32_t> members, blocks;
         for (size_t r = 0; r < R; ++r) {
             const ghip_sim *s = b->sims[r];
-            const bool take = op == kOpDecay ? s->has_mot
-                              : op_is_population(op) ? s->has_pop
-                                                     : s->tracks_moments;
+            // a lattice that divides or dies checks for drift in its plan
+            const bool take = op == kOpDecay           ? s->has_mot
+                              : op_is_population(op)   ? s->has_pop
+                              : op == kOpCheckRecentre ? s->tracks_moments && !s->has_pop
+                                                       : s->tracks_moments;
             if (!take) continue;
             members.push_back((uint32_t)r);
             blocks.push_back(cell_blocks(s, op));

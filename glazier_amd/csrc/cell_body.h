@@ -192,31 +192,35 @@ __device__ __forceinline__ void recentre_body(const CellArgs &a, int gate, const
     for (int q = 1; q < a.cw; ++q) s[q] = 0;
 }
 
-// Whether some live label's centroid sits more than an eighth of a side from
-// its anchor; sets flags[1].
-__device__ __forceinline__ void check_recentre_body(const CellArgs &a, const int lb, const int gb) {
-    uint32_t l = lb * blockDim.x + threadIdx.x;
-    if (l == 0 || l >= *a.n_labels) return;
-    const int64_t *s = a.state + (size_t)l * a.cw;
-    int64_t n = unpack_vol(s[0]);
-    if (n <= 0) return;
+// Whether a live label's centroid (first sums `s`, volume n > 0) sits more
+// than an eighth of a side from its anchor.
+__device__ __forceinline__ bool drifted(const CellArgs &a, const int64_t *s, int64_t n) {
     const int span[3] = {a.w, a.h, a.d};
     for (int ax = 0; ax < 3; ++ax) {
         if (span[ax] <= 1) continue;
         int64_t m = s[1 + ax];
         int64_t am = m < 0 ? -m : m;
-        if (8 * am > (int64_t)span[ax] * n) {
-            a.flags[1] = 1;
-            return;
-        }
+        if (8 * am > (int64_t)span[ax] * n) return true;
     }
+    return false;
+}
+
+// Sets flags[1] when some live label has drifted from its anchor.
+__device__ __forceinline__ void check_recentre_body(const CellArgs &a, const int lb, const int gb) {
+    uint32_t l = lb * blockDim.x + threadIdx.x;
+    if (l == 0 || l >= *a.n_labels) return;
+    const int64_t *s = a.state + (size_t)l * a.cw;
+    int64_t n = unpack_vol(s[0]);
+    if (n > 0 && drifted(a, s, n)) a.flags[1] = 1;
 }
 
 // ---------------------------------------------------------------- population
 
 // Decide each live label's fate: divide at its division volume (cut through
 // the centroid across the major axis), else die with the type's rate by a
-// counter draw of (step, label). Sets flags[0] when anything happens.
+// counter draw of (step, label). Sets flags[0] when anything happens, and
+// flags[1] when some label has drifted from its anchor (the recentring that
+// follows the step's division and death).
 __device__ __forceinline__ void plan_body(const CellArgs &a, const int lb, const int gb) {
     uint32_t l = lb * blockDim.x + threadIdx.x;
     uint32_t n = *a.n_labels;
@@ -226,6 +230,7 @@ __device__ __forceinline__ void plan_body(const CellArgs &a, const int lb, const
     const int64_t *s = a.state + (size_t)l * a.cw;
     int64_t v = unpack_vol(s[0]);
     if (v <= 0) return;
+    if (drifted(a, s, v)) a.flags[1] = 1;
     uint32_t t = label_type(a.ltype[l]);
     const DeviceTables *tab = a.tables;
     if (tab->div_vol[t] > 0.0 && (double)v >= tab->div_vol[t] && v >= 4) {
@@ -315,18 +320,45 @@ __device__ __forceinline__ void assign_daughters_body(const CellArgs &a, const i
             } else {
                 a.ltype[id] = (a.ltype[l] & ~kIdMask) | id; // the daughter keeps the type
                 a.daughter[l] = id;
+                a.plan[id] = 0;
                 a.anchor[3 * id] = a.anchor[3 * l];
                 a.anchor[3 * id + 1] = a.anchor[3 * l + 1];
                 a.anchor[3 * id + 2] = a.anchor[3 * l + 2];
+                // a fresh id: its record is recounted from its sites, and no
+                // pass delta left from an earlier owner of the id survives
+                for (int q = 0; q < a.cw; ++q) {
+                    a.state[(size_t)id * a.cw + q] = 0;
+                    for (int b = 0; b < 3; ++b) a.deltas[b][(size_t)id * a.cw + q] = 0;
+                }
             }
         }
+        // a dying label's record clears; a dividing one is recounted
+        if (l < n0 && l != 0 && a.plan[l] != 0)
+            for (int q = 0; q < a.cw; ++q) a.state[(size_t)l * a.cw + q] = 0;
         base += taken;
     }
     if (threadIdx.x == 0) *a.n_labels = base < a.capacity ? base : a.capacity;
 }
 
-// Relabel: a dying label's sites become medium, a dividing label's sites
-// beyond the cut go to its daughter.
+// A site's label once the step's division and death are applied: medium for
+// a dying label, the daughter beyond a dividing label's cut. Applying it to
+// a label already applied changes nothing (the medium and a daughter plan
+// nothing), so it reads the same whether or not the site was rewritten yet.
+__device__ __forceinline__ uint32_t label_after(const CellArgs &a, uint32_t l, int x, int y, int z) {
+    if (l == 0) return 0;
+    const uint8_t p = a.plan[l];
+    if (p == 2) return 0;
+    if (p == 1 && beyond_cut(a, l, x, y, z)) return a.daughter[l];
+    return l;
+}
+
+// Relabel, and recount the labels the step touched: a dying label's sites
+// become medium (its record was cleared and nobody else's changes: a
+// neighbour's bond to such a site is unlike before and after), a dividing
+// label's sites beyond the cut go to its daughter, and every site of the
+// parent or the daughter adds its volume, unlike bonds (against the
+// neighbours' labels after the step), moment sums about the shared anchor
+// and field sums to the record it now belongs to.
 __device__ __forceinline__ void apply_population_body(const CellArgs &a, size_t n_sites, const int lb, const int gb) {
     if (a.flags[0] == 0) return;
     size_t i = (size_t)lb * blockDim.x + threadIdx.x;
@@ -342,8 +374,29 @@ __device__ __forceinline__ void apply_population_body(const CellArgs &a, size_t 
     }
     int x, y, z;
     coords(i, a.w, a.h, &x, &y, &z);
-    if (beyond_cut(a, l, x, y, z)) {
-        a.labels[i] = (word & ~kIdMask) | a.daughter[l];
+    const uint32_t nl = beyond_cut(a, l, x, y, z) ? a.daughter[l] : l;
+    if (nl != l) a.labels[i] = (word & ~kIdMask) | nl;
+    int bonds = 0;
+    for (int k = 0; k < a.n_off; ++k) {
+        const int nx = wrap1(x + a.off[k][0], a.w), ny = wrap1(y + a.off[k][1], a.h),
+                  nz = wrap1(z + a.off[k][2], a.d);
+        const size_t nb = ((size_t)nz * a.h + ny) * (size_t)a.w + nx;
+        if (nb != i && label_after(a, label_id(a.labels[nb]), nx, ny, nz) != nl) ++bonds;
+    }
+    unsigned long long *rec = reinterpret_cast<unsigned long long *>(a.state + (size_t)nl * a.cw);
+    atomicAdd(rec, (unsigned long long)pack_vs(1, bonds));
+    if (a.cw > 1) {
+        const int64_t ux = fold_int(x - a.anchor[3 * nl], a.w);
+        const int64_t uy = fold_int(y - a.anchor[3 * nl + 1], a.h);
+        const int64_t uz = fold_int(z - a.anchor[3 * nl + 2], a.d);
+        const int64_t c[9] = {ux, uy, uz, ux * ux, uy * uy, uz * uz, ux * uy, ux * uz, uy * uz};
+#pragma unroll
+        for (int q = 0; q < 9; ++q)
+            if (c[q] != 0) atomicAdd(rec + 1 + q, (unsigned long long)c[q]);
+        if (a.cw > 10) {
+            atomicAdd(rec + 10, (unsigned long long)(int64_t)a.nematic[2 * i]);
+            atomicAdd(rec + 11, (unsigned long long)(int64_t)a.nematic[2 * i + 1]);
+        }
     }
 }
 
@@ -401,21 +454,20 @@ __device__ __forceinline__ void compact_move_body(const CellArgs &a, const uint3
     a.anchor[3 * to] = anchor_copy[3 * l];
     a.anchor[3 * to + 1] = anchor_copy[3 * l + 1];
     a.anchor[3 * to + 2] = anchor_copy[3 * l + 2];
+    for (int q = 0; q < a.cw; ++q) a.state[(size_t)to * a.cw + q] = a.state_copy[(size_t)l * a.cw + q];
 }
 
+// Relabels the sites with the compaction's remap; the first thread also sets
+// the new label count (nothing in this launch reads it).
 __device__ __forceinline__ void compact_relabel_body(const CellArgs &a, size_t n_sites, const int lb, const int gb) {
     if (a.flags[0] == 0 || a.flags[3] == 0) return;
     size_t i = (size_t)lb * blockDim.x + threadIdx.x;
+    if (i == 0) *a.n_labels = a.flags[3];
     if (i >= n_sites) return;
     uint32_t word = a.labels[i];
     uint32_t l = label_id(word);
     if (l == 0) return;
     a.labels[i] = (word & ~kIdMask) | a.scratch_u32[l];
-}
-
-__device__ __forceinline__ void compact_finish_body(const CellArgs &a, const int lb, const int gb) {
-    if (a.flags[0] == 0 || a.flags[3] == 0) return;
-    if (lb == 0 && threadIdx.x == 0) *a.n_labels = a.flags[3];
 }
 
 // Length minus target of every label in the snapshot a + e: the length
@@ -432,6 +484,19 @@ __device__ __forceinline__ void length_prepass_body(const int64_t *a, const int6
     const int64_t n = unpack_vol(a[base] + e[base]);
     const uint32_t ty = label_type(ltype[l]);
     out[l] = length_of(m, n, d3) - tables->tgt_len[ty];
+}
+
+// Staging copies for a compaction: types, anchors and records of the labels
+// in use, read back by compact_move_body.
+__device__ __forceinline__ void compact_stage_body(const CellArgs &a, uint32_t *type_copy, int32_t *anchor_copy,
+                                                   const int lb, const int gb) {
+    if (a.flags[0] == 0 || a.flags[3] == 0) return;
+    const size_t n = *a.n_labels, words = n * (size_t)a.cw, span = words > 3 * n ? words : 3 * n;
+    for (size_t i = (size_t)lb * blockDim.x + threadIdx.x; i < span; i += (size_t)gb * blockDim.x) {
+        if (i < words) a.state_copy[i] = a.state[i];
+        if (i < n) type_copy[i] = a.ltype[i];
+        if (i < 3 * n) anchor_copy[i] = a.anchor[i];
+    }
 }
 
 __device__ __forceinline__ void copy_u32_body(uint32_t *dst, const uint32_t *src, const uint32_t *n_labels, int per, const int lb, const int gb) {

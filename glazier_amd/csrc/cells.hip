@@ -80,22 +80,16 @@ __global__ void compact_relabel_kernel(CellArgs a, size_t n_sites) {
     compact_relabel_body(a, n_sites, blockIdx.x, gridDim.x);
 }
 
-__global__ void compact_finish_kernel(CellArgs a) {
-    compact_finish_body(a, blockIdx.x, gridDim.x);
-}
 
 __global__ void copy_u32_kernel(uint32_t *dst, const uint32_t *src, const uint32_t *n_labels, int per) {
     copy_u32_body(dst, src, n_labels, per, blockIdx.x, gridDim.x);
 }
 
-// The compaction's copies of the label types (one word a label) and anchors
-// (three) in one launch, only on a step whose plan compacts.
-__global__ void copy_types_anchors_kernel(uint32_t *types_dst, const uint32_t *types_src,
-                                          uint32_t *anchors_dst, const uint32_t *anchors_src,
-                                          const uint32_t *n_labels, const uint32_t *flags) {
-    if (flags[0] == 0 || flags[3] == 0) return;
-    copy_u32_body(types_dst, types_src, n_labels, 1, blockIdx.x, gridDim.x);
-    copy_u32_body(anchors_dst, anchors_src, n_labels, 3, blockIdx.x, gridDim.x);
+// The compaction's staging copies of the label types (one word a label),
+// anchors (three) and records (cw int64) in one launch, only on a step whose
+// plan compacts.
+__global__ void compact_stage_kernel(CellArgs a, uint32_t *type_copy, int32_t *anchor_copy) {
+    compact_stage_body(a, type_copy, anchor_copy, blockIdx.x, gridDim.x);
 }
 
 __global__ void length_prepass_kernel(const int64_t *a, const int64_t *e, const uint32_t *n_labels, int cw,
@@ -183,8 +177,10 @@ hipError_t launch_recentre_check(const CellArgs &a, size_t n_sites, hipStream_t 
     return hipGetLastError();
 }
 
-// Division and death of one step, then recounts, compaction and recentring,
-// every stage gated on the device flags so nothing returns to the host.
+// Division and death of one step (records of the touched labels recounted
+// in the relabelling pass), compaction with the records moved along, then
+// the recentring the plan's drift check asked for; every stage gated on the
+// device flags so nothing returns to the host.
 hipError_t launch_population(const CellArgs &a, size_t n_sites, uint32_t *type_copy,
                              int32_t *anchor_copy, hipStream_t s) {
     const int lb = blocks_for(a.capacity, 256);
@@ -193,23 +189,16 @@ hipError_t launch_population(const CellArgs &a, size_t n_sites, uint32_t *type_c
     hipLaunchKernelGGL(count_moved_kernel, dim3(sb), dim3(256), 0, s, a, n_sites);
     hipLaunchKernelGGL(assign_daughters_kernel, dim3(1), dim3(1024), 0, s, a);
     hipLaunchKernelGGL(apply_population_kernel, dim3(sb), dim3(256), 0, s, a, n_sites);
-    zero_state(a, 0, s);
-    recount_gated(a, n_sites, 0, 0, s);
     hipLaunchKernelGGL(compact_plan_kernel, dim3(1), dim3(1024), 0, s, a);
     {
-        int cb = blocks_for(a.capacity, 256);
+        int cb = blocks_for((size_t)a.capacity * a.cw, 256);
         if (cb > 4096) cb = 4096;
-        hipLaunchKernelGGL(copy_types_anchors_kernel, dim3(cb), dim3(256), 0, s, type_copy, a.ltype,
-                           reinterpret_cast<uint32_t *>(anchor_copy),
-                           reinterpret_cast<const uint32_t *>(a.anchor), a.n_labels, a.flags);
+        hipLaunchKernelGGL(compact_stage_kernel, dim3(cb), dim3(256), 0, s, a, type_copy, anchor_copy);
     }
     hipLaunchKernelGGL(compact_move_kernel, dim3(lb), dim3(256), 0, s, a, type_copy, anchor_copy);
     hipLaunchKernelGGL(compact_relabel_kernel, dim3(sb), dim3(256), 0, s, a, n_sites);
-    hipLaunchKernelGGL(compact_finish_kernel, dim3(1), dim3(1), 0, s, a);
-    zero_state(a, 3, s);
-    recount_gated(a, n_sites, 0, 3, s);
-    hipLaunchKernelGGL(recentre_kernel, dim3(lb), dim3(256), 0, s, a, 0);
-    recount_gated(a, n_sites, 1, 0, s);
+    hipLaunchKernelGGL(recentre_kernel, dim3(lb), dim3(256), 0, s, a, 1);
+    recount_gated(a, n_sites, 1, 1, s);
     return hipGetLastError();
 }
 

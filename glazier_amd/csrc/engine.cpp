@@ -409,9 +409,11 @@ ghip_status run_cells(ghip_sim *s) {
     if (s->has_pop) {
         c.key = pass_key(s->m.seed, s->m.replica, s->mcs, 0xDEADCE11U);
         c.key2 = second_key(c.key);
+        // the plan checks for drift; the chain ends with the recentring
         HIP_TRY(launch_population(c, s->N, s->type_copy, s->anchor_copy, s->stream));
+    } else {
+        HIP_TRY(launch_recentre_check(c, s->N, s->stream));
     }
-    HIP_TRY(launch_recentre_check(c, s->N, s->stream));
     HIP_TRY(s->end_timed(t));
     return GHIP_OK;
 }

@@ -175,6 +175,8 @@ struct CellArgs {
     double *geo;           // 6 per label
     const int32_t *nematic; // (Q_xx, Q_xy) per site in units of 2^-24, or null
     uint32_t *flags;       // [0] event, [1] recentre, [2] overflow, [3] compacted count
+    int64_t *state_copy;   // staging for records moved by a compaction (the other base buffer)
+    int64_t *deltas[3];    // the pass delta buffers, cleared for a daughter's fresh id
     const DeviceTables *tables;
     int w, h, d;
     int cw;

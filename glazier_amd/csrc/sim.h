@@ -145,6 +145,8 @@ struct ghip_sim {
         c.geo = geo;
         c.nematic = nematic;
         c.flags = flags;
+        c.state_copy = A[(pass + 1) % 2];
+        for (int i = 0; i < 3; ++i) c.deltas[i] = Dl[i];
         c.tables = tables;
         c.w = W;
         c.h = H;

@@ -957,6 +957,8 @@ static uint32_t death_draw(const emu_sim *e, uint32_t label) {
     return oracle_hash32(oracle_hash32(label ^ key) ^ second_key(key));
 }
 
+/* Compaction: live labels renumbered densely in ascending order, each taking
+ * its type, anchor and records along. */
 static void compact(emu_sim *e) {
     uint32_t *remap = malloc(e->n_labels * 4);
     size_t next = 1;
@@ -965,9 +967,11 @@ static void compact(emu_sim *e) {
         if (e->vol[l] > 0) {
             remap[l] = (uint32_t)next;
             e->ctype[next] = e->ctype[l];
-            e->anchor[3 * next] = e->anchor[3 * l];
-            e->anchor[3 * next + 1] = e->anchor[3 * l + 1];
-            e->anchor[3 * next + 2] = e->anchor[3 * l + 2];
+            memmove(e->anchor + 3 * next, e->anchor + 3 * l, 3 * 4);
+            e->vol[next] = e->vol[l];
+            e->surf[next] = e->surf[l];
+            memmove(e->mom + 9 * next, e->mom + 9 * l, 9 * 8);
+            memmove(e->nem + 2 * next, e->nem + 2 * l, 2 * 8);
             ++next;
         } else {
             remap[l] = 0;
@@ -978,6 +982,23 @@ static void compact(emu_sim *e) {
     free(remap);
 }
 
+/* Which side of its parent's cut a site of a dividing label falls on. */
+static int beyond_cut(const emu_sim *e, const double *geo, uint32_t l, int64_t x, int64_t y, int64_t z) {
+    double ux = (double)fold_int(x - e->anchor[3 * l], e->w);
+    double uy = (double)fold_int(y - e->anchor[3 * l + 1], e->h);
+    double uz = (double)fold_int(z - e->anchor[3 * l + 2], e->d);
+    const double *g = geo + 6 * l;
+    double along = (ux - g[0]) * g[3] + (uy - g[1]) * g[4] + (uz - g[2]) * g[5];
+    return along > 0.0;
+}
+
+/* Division and death of one step. Only the labels an event touches change
+ * their records: a dying label's are cleared (its sites become medium, which
+ * leaves every other label's surface as it was: a neighbour's bond to the
+ * site was unlike before and stays unlike), and a dividing label and its
+ * daughter are recounted from their sites, with the daughter taking the
+ * parent's anchor. Other labels' records stay exact, so there is no full
+ * recount; compaction moves records with their labels. */
 static void population(emu_sim *e) {
     int three = e->d > 1;
     size_t n0 = e->n_labels;
@@ -1015,12 +1036,7 @@ static void population(emu_sim *e) {
         if (l == 0 || plan[l] != 1) continue;
         int64_t x, y, z;
         e_coords(e, i, &x, &y, &z);
-        double ux = (double)fold_int(x - e->anchor[3 * l], e->w);
-        double uy = (double)fold_int(y - e->anchor[3 * l + 1], e->h);
-        double uz = (double)fold_int(z - e->anchor[3 * l + 2], e->d);
-        const double *g = geo + 6 * l;
-        double along = (ux - g[0]) * g[3] + (uy - g[1]) * g[4] + (uz - g[2]) * g[5];
-        if (along > 0.0) moved[l] += 1;
+        if (beyond_cut(e, geo, l, x, y, z)) moved[l] += 1;
     }
     uint32_t *daughter = calloc(n0, 4);
     size_t next = n0;
@@ -1033,47 +1049,80 @@ static void population(emu_sim *e) {
         daughter[l] = (uint32_t)next++;
     }
     ensure_cap(e, next);
-    for (size_t l = 1; l < n0; ++l)
-        if (plan[l] == 1) {
-            uint32_t dl = daughter[l];
-            e->ctype[dl] = e->ctype[l];
-            e->anchor[3 * dl] = e->anchor[3 * l];
-            e->anchor[3 * dl + 1] = e->anchor[3 * l + 1];
-            e->anchor[3 * dl + 2] = e->anchor[3 * l + 2];
-        }
+    /* new label of every site, from its old one (a daughter keeps its id) */
+    uint32_t *fresh = malloc(e->n * 4);
     for (size_t i = 0; i < e->n; ++i) {
         uint32_t l = e->labels[i];
+        fresh[i] = l;
         if (l == 0 || plan[l] == 0) continue;
         if (plan[l] == 2) {
-            e->labels[i] = 0;
+            fresh[i] = 0;
             continue;
         }
         int64_t x, y, z;
         e_coords(e, i, &x, &y, &z);
-        double ux = (double)fold_int(x - e->anchor[3 * l], e->w);
-        double uy = (double)fold_int(y - e->anchor[3 * l + 1], e->h);
-        double uz = (double)fold_int(z - e->anchor[3 * l + 2], e->d);
-        const double *g = geo + 6 * l;
-        double along = (ux - g[0]) * g[3] + (uy - g[1]) * g[4] + (uz - g[2]) * g[5];
-        if (along > 0.0) e->labels[i] = daughter[l];
+        if (beyond_cut(e, geo, l, x, y, z)) fresh[i] = daughter[l];
     }
+    for (size_t l = 1; l < n0; ++l) {
+        if (plan[l] == 0) continue;
+        size_t ids[2] = {l, plan[l] == 1 ? daughter[l] : 0};
+        for (int q = 0; q < 2; ++q) {
+            size_t k = ids[q];
+            if (q == 1 && plan[l] != 1) break;
+            e->vol[k] = 0;
+            e->surf[k] = 0;
+            memset(e->mom + 9 * k, 0, 9 * 8);
+            memset(e->nem + 2 * k, 0, 2 * 8);
+        }
+        if (plan[l] == 1) {
+            uint32_t dl = daughter[l];
+            e->ctype[dl] = e->ctype[l];
+            memcpy(e->anchor + 3 * dl, e->anchor + 3 * l, 3 * 4);
+        }
+    }
+    /* recount the parents and daughters from their sites */
+    for (size_t i = 0; i < e->n; ++i) {
+        uint32_t old = e->labels[i];
+        if (old == 0 || plan[old] != 1) continue;
+        uint32_t l = fresh[i];
+        int64_t x, y, z;
+        e_coords(e, i, &x, &y, &z);
+        e->vol[l] += 1;
+        for (int k = 0; k < e->n_off; ++k) {
+            size_t nb = e_index(e, x + e->off[k][0], y + e->off[k][1], z + e->off[k][2]);
+            if (nb != i && fresh[nb] != l) e->surf[l] += 1;
+        }
+        int64_t ux = fold_int(x - e->anchor[3 * l], e->w);
+        int64_t uy = fold_int(y - e->anchor[3 * l + 1], e->h);
+        int64_t uz = fold_int(z - e->anchor[3 * l + 2], e->d);
+        int64_t *m = e->mom + 9 * l;
+        m[0] += ux;
+        m[1] += uy;
+        m[2] += uz;
+        m[3] += ux * ux;
+        m[4] += uy * uy;
+        m[5] += uz * uz;
+        m[6] += ux * uy;
+        m[7] += ux * uz;
+        m[8] += uy * uz;
+        if (e->has_nem) {
+            e->nem[2 * l] += e->nq[2 * i];
+            e->nem[2 * l + 1] += e->nq[2 * i + 1];
+        }
+    }
+    memcpy(e->labels, fresh, e->n * 4);
     e->n_labels = next;
+    free(fresh);
     free(plan);
     free(geo);
     free(moved);
     free(daughter);
-    recount_volume_surface(e);
     size_t dead = 0;
     for (size_t l = 1; l < e->n_labels; ++l)
         if (e->vol[l] <= 0) ++dead;
     size_t limit = (e->n_labels - 1) / 4;
     if (limit < 64) limit = 64;
-    if (dead > limit) {
-        compact(e);
-        recount_volume_surface(e);
-    }
-    sum_moments(e);
-    recentre(e);
+    if (dead > limit) compact(e);
 }
 
 /* ---------------------------------------------------------------- step */
@@ -1086,8 +1135,11 @@ void emu_step(emu_sim *e, uint64_t steps) {
         if (e->m.n_species > 0) emu_fields_step(e);
         if (e->has_mot)
             for (size_t i = 0; i < e->n; ++i) e->act[i] = e->act[i] > 0.0f ? e->act[i] - 1.0f : 0.0f;
+        /* the drift check reads the records the step's plan reads, before
+         * division and death; the recentring follows them */
+        const int drifted = e->tracks_moments && needs_recentre(e);
         if (e->has_pop) population(e);
-        if (e->tracks_moments && needs_recentre(e)) recentre(e);
+        if (drifted) recentre(e);
         e->mcs += 1;
     }
 }

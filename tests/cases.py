@@ -152,6 +152,12 @@ def case(name):
         m.types[1].target_volume = 100.0
         m.types[1].death_rate = 0.005
         return (m, *tiled(m, side=6, nx=6, ny=6), None, None, 40)
+    if name == "compaction":
+        # many deaths, so the dead labels pass a quarter of the count and
+        # are compacted away while cells keep dividing into fresh ids
+        m = ref_model(64, 64, seed=74, division_volume=20.0, death_rate=0.06)
+        m.types[1].target_volume = 24.0
+        return (m, *tiled(m, side=4), None, None, 24)
     if name == "infection128":
         m, l, t = infection_sheet(128, 7)
         return m, l, t, None, None, 30
@@ -199,5 +205,5 @@ PARITY_CASES = ["plain128", "surface256", "surface1024", "scattered1024", "fast_
                 "fast_lone_typed", "ref64", "medium_gaps", "rect64x32",
                 "von_neumann", "float_energies", "volume3d_o2", "volume3d_o3", "volume3d_o1",
                 "connected", "connected3d", "chemotaxis_ramp", "secreting", "length",
-                "motility", "drift", "death", "division", "infection128", "motility3d",
+                "motility", "drift", "death", "division", "compaction", "infection128", "motility3d",
                 "length3d", "division3d", "nematic", "nematic_division", "nematic3d"]
