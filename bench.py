@@ -281,12 +281,7 @@ def configs_bench():
                 "acceptance": a, "seconds": el,
                 "hbm_frac_algorithmic": attempts * bytes_per_attempt(a) / el / (HBM_PEAK_GBS * 1e9)}
 
-    def reference_model(side):
-        c = CellType(target_volume=64.0, lambda_volume=2.0)
-        m = Model(width=side, height=side, contact=[0.0, 16.0, 16.0, 8.0], types=[CellType(), c],
-                  temperature=10.0, neighbour_order=2, seed=1)
-        n = side // 8
-        return m, tiled_labels(m, 8, n, n, 1), default_cell_types(n * n + 1)
+    from glazier_amd.workloads import reference_model
 
     plain = lambda a: 16.0 + 4.0 * a
     # infection: 4 colours x (labels + the chemotactic field) + accepted
@@ -300,7 +295,7 @@ def configs_bench():
     # README.md:116-121, site-attempts/s derived from the listed GPU seconds
     published = {128: 8.2e8, 256: 3.3e9, 512: 3.7e9, 1024: 4.4e9, 2048: 4.3e9, 4096: 3.76e9}
     for side, ref in published.items():
-        r = run([reference_model(side)], 100, plain)
+        r = run([reference_model(side, 1)], 100, plain)
         r["reference_published_site_attempts_per_s"] = ref
         r["ratio_to_reference"] = r["site_attempts_per_s"] / ref
         out[f"reference_model_{side}"] = r
@@ -481,6 +476,10 @@ def main() -> int:
                     help="skip the rocprofv3 counter runs that measure the pass's HBM bytes")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="only run a few passes of one lattice (the counter runs use this)")
+    ap.add_argument("--probe", choices=["config2", "moving4096"], default=None,
+                    help="only step one workload and print its step time, for a profiler run: "
+                         "config2 (512^2 infection sheet, seed 7, 200 steps) or moving4096 "
+                         "(the reference's benchmark model at 4096^2, 25 steps)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy a MIB-mebibyte buffer 5 times with torch, so a counter "
                          "run can scale FETCH_SIZE/WRITE_SIZE by a copy of known size")
@@ -505,6 +504,22 @@ def main() -> int:
     from glazier_amd.workloads import plain_monolayer
 
     side = args.side
+    if args.probe:
+        from glazier_amd.workloads import infection_sheet, reference_model
+
+        built, steps = ((infection_sheet(512, 7), 200) if args.probe == "config2"
+                        else (reference_model(4096, 1), 25))
+        sim = HipSimulation(*built)
+        sim.step(1)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        sim.step(steps)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        print(json.dumps({"probe": args.probe, "steps": steps, "us_per_step": el / steps * 1e6,
+                          "n_labels": sim.n_labels()}))
+        sim.close()
+        return 0
     if args.traffic_probe:
         model, labels, types = plain_monolayer(side, seed=1, surface=True)
         sim = HipSimulation(model, labels, types)

@@ -36,6 +36,8 @@ struct BatchReplica {
     uint32_t n_sites;
     int colours;
     int has_mot;
+    int has_pop;
+    uint32_t *flags_base;  // both sets of flag words
     int n_species;
     int lab0, act0;              // current label and activity buffers at mcs0
     int field0[kMaxSpecies];     // current buffer of each species at mcs0
@@ -68,16 +70,17 @@ struct WorkList {
 };
 
 // The per-label launches of a step, in the order the single-lattice driver
-// enqueues them (consolidation with the flag clear, division/death whose
-// plan checks for drift, compaction, then the recentring; lattices without
-// division or death check for drift on their own), plus the activity decay.
+// enqueues them: for a lattice that divides or dies, the plan (which folds
+// the last pass's deltas, clears the next step's flags and checks for
+// drift), daughters (with the compaction plan), relabelling, compaction;
+// for one that only keeps moments, consolidation (with the flag clear) and
+// the drift check; then the recentring, and the activity decay.
 enum BatchCellOp : int {
     kOpConsolidate,
     kOpPlan,
     kOpCountMoved,
     kOpAssign,
     kOpApply,
-    kOpCompactPlan,
     kOpCompactStage,
     kOpCompactMove,
     kOpCompactRelabel,

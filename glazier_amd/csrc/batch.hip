@@ -171,6 +171,11 @@ __global__ void batch_prepare_cells_kernel(BatchArgs b) {
     const uint64_t p_end = passes_before(R, m + 1);
     c.state = R.A[p_end % 2];
     c.state_copy = R.A[(p_end + 1) % 2];
+    // a lattice that divides or dies alternates two sets of flag words
+    if (R.has_pop) {
+        c.flags = R.flags_base + 4 * (m & 1);
+        c.flags_next = R.flags_base + 4 * ((m + 1) & 1);
+    }
     c.labels = R.labels[flip_index(R, R.lab0, p_end)];
     c.key = pass_key(R.seed, R.replica, m, 0xDEADCE11U);
     c.key2 = second_key(c.key);
@@ -201,15 +206,14 @@ __global__ __launch_bounds__(THREADS) void batch_cell_kernel(BatchArgs b, WorkLi
         // the moved-site counts the next stage accumulates start at zero
         const uint32_t i = lb * blockDim.x + threadIdx.x;
         if (i < c.capacity) c.scratch_u32[i] = 0u;
-        plan_body(c, lb, gb);
+        const uint64_t p = passes_before(R, b.mcs[r] + 1);
+        plan_body(c, R.D[(p + 2) % 3], lb, gb);
     } else if constexpr (OP == kOpCountMoved) {
         count_moved_body(c, n, lb, gb);
     } else if constexpr (OP == kOpAssign) {
         assign_daughters_body(c, lb, gb);
     } else if constexpr (OP == kOpApply) {
         apply_population_body(c, n, lb, gb);
-    } else if constexpr (OP == kOpCompactPlan) {
-        compact_plan_body(c, lb, gb);
     } else if constexpr (OP == kOpCompactStage) {
         compact_stage_body(c, R.type_copy, R.anchor_copy, lb, gb);
     } else if constexpr (OP == kOpCompactMove) {
@@ -294,7 +298,7 @@ hipError_t launch_batch_advance(const BatchArgs &b, hipStream_t s) {
     return hipGetLastError();
 }
 
-int batch_cell_threads(int op) { return (op == kOpAssign || op == kOpCompactPlan) ? 1024 : 256; }
+int batch_cell_threads(int op) { return op == kOpAssign ? 1024 : 256; }
 
 hipError_t launch_batch_cell(int op, const BatchArgs &b, const WorkList &w, hipStream_t s) {
     if (w.blocks == 0) return hipSuccess;
@@ -308,7 +312,6 @@ hipError_t launch_batch_cell(int op, const BatchArgs &b, const WorkList &w, hipS
         GHIP_BC(kOpCountMoved, 256);
         GHIP_BC(kOpAssign, 1024);
         GHIP_BC(kOpApply, 256);
-        GHIP_BC(kOpCompactPlan, 1024);
         GHIP_BC(kOpCompactStage, 256);
         GHIP_BC(kOpCompactMove, 256);
         GHIP_BC(kOpCompactRelabel, 256);

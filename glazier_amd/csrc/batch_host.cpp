@@ -60,7 +60,6 @@ uint32_t cell_blocks(const ghip_sim *s, int op) {
         case kOpCompactStage:
             return std::min<uint32_t>(4096, cdiv(words, 256));
         case kOpAssign:
-        case kOpCompactPlan:
             return 1;
         case kOpCountMoved:
         case kOpApply:
@@ -250,6 +249,8 @@ ghip_status rebuild(ghip_batch *b) {
         B.n_sites = (uint32_t)s->N;
         B.colours = s->colours;
         B.has_mot = s->has_mot ? 1 : 0;
+        B.has_pop = s->has_pop ? 1 : 0;
+        B.flags_base = s->flags;
         B.n_species = (int)s->m.n_species;
         B.lab0 = s->lab_cur;
         B.act0 = s->act_cur;
@@ -304,7 +305,7 @@ ghip_status rebuild(ghip_batch *b) {
         std::vector<uint32_t> members, blocks;
         for (size_t r = 0; r < R; ++r) {
             const ghip_sim *s = b->sims[r];
-            if (!(s->sweep_flags & kLength)) continue;
+            if (!(s->sweep_flags & kLength) || !s->d3) continue;  // planes price it inline
             members.push_back((uint32_t)r);
             blocks.push_back(cdiv(s->cap, 256));
         }
@@ -319,8 +320,9 @@ ghip_status rebuild(ghip_batch *b) {
             // a lattice that divides or dies checks for drift in its plan
             const bool take = op == kOpDecay           ? s->has_mot
                               : op_is_population(op)   ? s->has_pop
-                              : op == kOpCheckRecentre ? s->tracks_moments && !s->has_pop
-                                                       : s->tracks_moments;
+                              : op == kOpConsolidate || op == kOpCheckRecentre
+                                  ? s->tracks_moments && !s->has_pop
+                                  : s->tracks_moments;
             if (!take) continue;
             members.push_back((uint32_t)r);
             blocks.push_back(cell_blocks(s, op));
@@ -427,10 +429,11 @@ void advance_host(ghip_batch *b, uint64_t steps) {
 ghip_status check_capacity(ghip_batch *b) {
     for (ghip_sim *s : b->sims) {
         if (!s->has_pop) continue;
-        uint32_t fl[4];
-        HIP_TRY(hipMemcpy(fl, s->flags, 16, hipMemcpyDeviceToHost));
-        if (fl[2]) {
+        uint32_t fl[8];
+        HIP_TRY(hipMemcpy(fl, s->flags, 32, hipMemcpyDeviceToHost));
+        if (fl[2] || fl[6]) {
             HIP_TRY(hipMemset(s->flags + 2, 0, 4));
+            HIP_TRY(hipMemset(s->flags + 6, 0, 4));
             return fail(GHIP_DEVICE, "a lattice in the batch reached the label id limit; "
                                      "divisions were dropped");
         }

@@ -221,10 +221,22 @@ __device__ __forceinline__ void check_recentre_body(const CellArgs &a, const int
 // counter draw of (step, label). Sets flags[0] when anything happens, and
 // flags[1] when some label has drifted from its anchor (the recentring that
 // follows the step's division and death).
-__device__ __forceinline__ void plan_body(const CellArgs &a, const int lb, const int gb) {
+// With `e`, the label's record is brought current first (a += e, e = 0:
+// the consolidation of the step's last pass, one label per thread), and the
+// first threads clear the next step's flag words (word 2 is kept for the
+// host).
+__device__ __forceinline__ void plan_body(const CellArgs &a, int64_t *e, const int lb, const int gb) {
     uint32_t l = lb * blockDim.x + threadIdx.x;
+    if (l < 4 && l != 2 && a.flags_next) a.flags_next[l] = 0u;
     uint32_t n = *a.n_labels;
     if (l >= n) return;
+    if (e) {
+        for (int q = 0; q < a.cw; ++q) {
+            const size_t w = (size_t)l * a.cw + q;
+            a.state[w] += e[w];
+            e[w] = 0;
+        }
+    }
     a.plan[l] = 0;
     if (l == 0) return;
     const int64_t *s = a.state + (size_t)l * a.cw;
@@ -294,6 +306,42 @@ __device__ __forceinline__ uint32_t block_rank(bool flag, uint32_t *wave_counts,
     return before + below;
 }
 
+// Tail of the daughter assignment (same workgroup, after its loop): when
+// more than max(64, (n-1)/4) of the n labels are dead once the step's deaths
+// and divisions are applied, renumber the live ones densely in ascending
+// order (remap into scratch_u32, whose moved-site counts are spent) and set
+// flags[3] to the new count. Live: a daughter (id >= n0), a dividing label,
+// and any other label with volume left and no death planned.
+__device__ __forceinline__ void plan_compaction(const CellArgs &a, uint32_t n0, uint32_t n, uint32_t *wave_counts) {
+    __shared__ uint32_t dead_total;
+    if (threadIdx.x == 0) dead_total = 0;
+    __syncthreads();
+    auto live = [&](uint32_t l) {
+        if (l >= n0) return true;
+        const uint8_t p = a.plan[l];
+        return p == 1 || (p == 0 && unpack_vol(a.state[(size_t)l * a.cw]) > 0);
+    };
+    for (uint32_t l = 1 + threadIdx.x; l < n; l += blockDim.x)
+        if (!live(l)) atomicAdd(&dead_total, 1u);
+    __syncthreads();
+    uint32_t limit = (n - 1) / 4;
+    if (limit < 64) limit = 64;
+    if (dead_total <= limit) return;
+    uint32_t next = 1;
+    for (uint32_t start = 1; start < n; start += blockDim.x) {
+        uint32_t l = start + threadIdx.x;
+        const bool keep = l < n && live(l);
+        uint32_t lived = 0;
+        const uint32_t rank = block_rank(keep, wave_counts, &lived);
+        if (l < n) a.scratch_u32[l] = keep ? next + rank : 0u;
+        next += lived;
+    }
+    if (threadIdx.x == 0) {
+        a.scratch_u32[0] = 0;
+        a.flags[3] = next; // the new label count, nonzero
+    }
+}
+
 __device__ __forceinline__ void assign_daughters_body(const CellArgs &a, const int lb, const int gb) {
     if (a.flags[0] == 0) return;
     __shared__ uint32_t wave_counts[16];
@@ -337,7 +385,10 @@ __device__ __forceinline__ void assign_daughters_body(const CellArgs &a, const i
             for (int q = 0; q < a.cw; ++q) a.state[(size_t)l * a.cw + q] = 0;
         base += taken;
     }
-    if (threadIdx.x == 0) *a.n_labels = base < a.capacity ? base : a.capacity;
+    const uint32_t n = base < a.capacity ? base : a.capacity;
+    if (threadIdx.x == 0) *a.n_labels = n;
+    __syncthreads();  // every moved-site count is read; the plans are final
+    plan_compaction(a, n0, n, wave_counts);
 }
 
 // A site's label once the step's division and death are applied: medium for
@@ -407,37 +458,6 @@ __device__ __forceinline__ void zero_state_body(const CellArgs &a, int gate, con
     for (size_t i = (size_t)lb * blockDim.x + threadIdx.x; i < n;
          i += (size_t)gb * blockDim.x)
         a.state[i] = 0;
-}
-
-// One workgroup: when more than max(64, (n-1)/4) labels are dead, renumber
-// the live ones densely in ascending order (remap into scratch_u32) and set
-// flags[3].
-__device__ __forceinline__ void compact_plan_body(const CellArgs &a, const int lb, const int gb) {
-    if (a.flags[0] == 0) return;
-    __shared__ uint32_t dead_total;
-    const uint32_t n = *a.n_labels;
-    if (threadIdx.x == 0) dead_total = 0;
-    __syncthreads();
-    for (uint32_t l = 1 + threadIdx.x; l < n; l += blockDim.x)
-        if (unpack_vol(a.state[(size_t)l * a.cw]) <= 0) atomicAdd(&dead_total, 1u);
-    __syncthreads();
-    uint32_t limit = (n - 1) / 4;
-    if (limit < 64) limit = 64;
-    if (dead_total <= limit) return;
-    __shared__ uint32_t wave_counts[16];
-    uint32_t next = 1;
-    for (uint32_t start = 1; start < n; start += blockDim.x) {
-        uint32_t l = start + threadIdx.x;
-        uint32_t live = (l < n && unpack_vol(a.state[(size_t)l * a.cw]) > 0) ? 1u : 0u;
-        uint32_t lived = 0;
-        const uint32_t rank = block_rank(live != 0, wave_counts, &lived);
-        if (l < n) a.scratch_u32[l] = live ? next + rank : 0u;
-        next += lived;
-    }
-    if (threadIdx.x == 0) {
-        a.scratch_u32[0] = 0;
-        a.flags[3] = next; // the new label count, nonzero
-    }
 }
 
 // Move types and anchors of live labels to their new ids (ascending, so a

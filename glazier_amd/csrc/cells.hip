@@ -45,11 +45,11 @@ __global__ void check_recentre_kernel(CellArgs a) {
     check_recentre_body(a, blockIdx.x, gridDim.x);
 }
 
-__global__ void plan_kernel(CellArgs a) {
+__global__ void plan_kernel(CellArgs a, int64_t *e) {
     // the moved-site counts the next stage accumulates start at zero
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < a.capacity) a.scratch_u32[i] = 0u;
-    plan_body(a, blockIdx.x, gridDim.x);
+    plan_body(a, e, blockIdx.x, gridDim.x);
 }
 
 __global__ void count_moved_kernel(CellArgs a, size_t n_sites) {
@@ -68,9 +68,6 @@ __global__ void zero_state_kernel(CellArgs a, int gate) {
     zero_state_body(a, gate, blockIdx.x, gridDim.x);
 }
 
-__global__ __launch_bounds__(1024) void compact_plan_kernel(CellArgs a) {
-    compact_plan_body(a, blockIdx.x, gridDim.x);
-}
 
 __global__ void compact_move_kernel(CellArgs a, const uint32_t *type_copy, const int32_t *anchor_copy) {
     compact_move_body(a, type_copy, anchor_copy, blockIdx.x, gridDim.x);
@@ -177,19 +174,20 @@ hipError_t launch_recentre_check(const CellArgs &a, size_t n_sites, hipStream_t 
     return hipGetLastError();
 }
 
-// Division and death of one step (records of the touched labels recounted
-// in the relabelling pass), compaction with the records moved along, then
-// the recentring the plan's drift check asked for; every stage gated on the
+// Division and death of one step: the plan (which first folds the last
+// pass's deltas into the records), the daughter assignment (which also plans
+// any compaction), the relabelling pass (which recounts the touched
+// labels), the compaction with the records moved along, then the recentring
+// the plan's drift check asked for; every stage after the plan gated on the
 // device flags so nothing returns to the host.
-hipError_t launch_population(const CellArgs &a, size_t n_sites, uint32_t *type_copy,
+hipError_t launch_population(const CellArgs &a, int64_t *last_deltas, size_t n_sites, uint32_t *type_copy,
                              int32_t *anchor_copy, hipStream_t s) {
     const int lb = blocks_for(a.capacity, 256);
     const int sb = blocks_for(n_sites, 256);
-    hipLaunchKernelGGL(plan_kernel, dim3(lb), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(plan_kernel, dim3(lb), dim3(256), 0, s, a, last_deltas);
     hipLaunchKernelGGL(count_moved_kernel, dim3(sb), dim3(256), 0, s, a, n_sites);
     hipLaunchKernelGGL(assign_daughters_kernel, dim3(1), dim3(1024), 0, s, a);
     hipLaunchKernelGGL(apply_population_kernel, dim3(sb), dim3(256), 0, s, a, n_sites);
-    hipLaunchKernelGGL(compact_plan_kernel, dim3(1), dim3(1024), 0, s, a);
     {
         int cb = blocks_for((size_t)a.capacity * a.cw, 256);
         if (cb > 4096) cb = 4096;

@@ -324,7 +324,7 @@ ghip_status run_sweep_pass(ghip_sim *s, int colour) {
     a.fold_words = s->has_pop ? 0u : (uint32_t)words;
     TimedLaunch t{};
     HIP_TRY(s->begin_timed(0, &t));
-    if (s->sweep_flags & kLength)
+    if ((s->sweep_flags & kLength) && s->d3)
         HIP_TRY(launch_length_prepass(a.a_cur, a.e_prev, s->n_labels_dev, s->cw, s->ltype, s->tables, s->d3,
                                       s->len_before, s->cap, s->stream));
     if (s->plain_fast) {
@@ -398,11 +398,13 @@ ghip_status run_fields(ghip_sim *s) {
 }
 
 ghip_status run_cells(ghip_sim *s) {
-    // bring the per-label state current: A[p%2] += D[(p+2)%3]
+    // the per-label state is brought current, A[p%2] += D[(p+2)%3]: by the
+    // division plan where there is one, else by a consolidation that also
+    // clears the step's flags
     const uint64_t p = s->pass;
-    // (and clear the step's flags)
-    HIP_TRY(launch_consolidate(s->A[p % 2], s->Dl[(p + 2) % 3], s->n_labels_dev, s->cw, s->cap,
-                               s->stream, s->flags));
+    if (!s->has_pop)
+        HIP_TRY(launch_consolidate(s->A[p % 2], s->Dl[(p + 2) % 3], s->n_labels_dev, s->cw, s->cap,
+                                   s->stream, s->flags));
     TimedLaunch t{};
     HIP_TRY(s->begin_timed(3, &t));
     CellArgs c = s->cell_args();
@@ -410,7 +412,7 @@ ghip_status run_cells(ghip_sim *s) {
         c.key = pass_key(s->m.seed, s->m.replica, s->mcs, 0xDEADCE11U);
         c.key2 = second_key(c.key);
         // the plan checks for drift; the chain ends with the recentring
-        HIP_TRY(launch_population(c, s->N, s->type_copy, s->anchor_copy, s->stream));
+        HIP_TRY(launch_population(c, s->Dl[(p + 2) % 3], s->N, s->type_copy, s->anchor_copy, s->stream));
     } else {
         HIP_TRY(launch_recentre_check(c, s->N, s->stream));
     }
@@ -677,7 +679,7 @@ ghip_status ghip_create_nematic(const ghip_model *model, const uint32_t *labels,
         CREATE_TRY(hipMalloc(&s->nematic, N * 2 * 4));
         CREATE_TRY(hipMemcpy(s->nematic, nq.data(), N * 2 * 4, hipMemcpyHostToDevice));
     }
-    CREATE_TRY(hipMalloc(&s->flags, 16));
+    CREATE_TRY(hipMalloc(&s->flags, 32));
     if (s->quiet_skip) {
         // every pass so far counts as having accepted copies (the second
         // record buffer does not hold the first's values yet)
@@ -718,7 +720,7 @@ ghip_status ghip_create_nematic(const ghip_model *model, const uint32_t *labels,
     CREATE_TRY(hipMemcpy(s->n_labels_dev, &nl, 4, hipMemcpyHostToDevice));
     CREATE_TRY(hipMemcpy(s->tables, &s->host_tables, sizeof(DeviceTables), hipMemcpyHostToDevice));
     CREATE_TRY(hipMemset(s->accepted, 0, kAcceptSlots * 8));
-    CREATE_TRY(hipMemset(s->flags, 0, 16));
+    CREATE_TRY(hipMemset(s->flags, 0, 32));
     for (int i = 0; i < 3; ++i) CREATE_TRY(hipMemset(s->Dl[i], 0, words * 8));
     CREATE_TRY(hipMemset(s->A[1], 0, words * 8));
     CREATE_TRY(hipMemset(s->anchor, 0, (size_t)cap * 12));
@@ -796,11 +798,13 @@ ghip_status ghip_step(ghip_sim *s, uint64_t steps) {
         s->launches[kStepFamily] += s->pass - pass0;
     }
     if (s->has_pop) {
-        // word 2 outlives the per-step clears: set by any step of the call
-        uint32_t fl[4];
-        HIP_TRY(hipMemcpy(fl, s->flags, 16, hipMemcpyDeviceToHost));
-        if (fl[2]) {
+        // word 2 of either set outlives the per-step clears: set by any
+        // step of the call
+        uint32_t fl[8];
+        HIP_TRY(hipMemcpy(fl, s->flags, 32, hipMemcpyDeviceToHost));
+        if (fl[2] || fl[6]) {
             HIP_TRY(hipMemset(s->flags + 2, 0, 4));
+            HIP_TRY(hipMemset(s->flags + 6, 0, 4));
             return fail(GHIP_DEVICE, "the label capacity ran out at the id limit; divisions were dropped");
         }
     }

@@ -12,7 +12,7 @@ __global__ __launch_bounds__(256) void field_kernel(FieldArgs a) {
 // Every species of a plane in one launch, K substeps each on 64 x 32 tiles:
 // block b takes tile b % tiles of species b / tiles.
 template <int K>
-__global__ __launch_bounds__(256) void field_planes_kernel(FieldPlanes p) {
+__global__ __launch_bounds__(1024) void field_planes_kernel(FieldPlanes p) {
     constexpr int total = (kFieldIX + 2 * K) * (kFieldIY + 2 * K);
     __shared__ float u[total], v[total];
     const int tiles = p.sp[0].tiles_x * p.sp[0].tiles_y;
@@ -51,11 +51,15 @@ hipError_t launch_field_planes(const FieldPlanes &p, hipStream_t stream) {
         f.iy_log2 != 5)
         return hipErrorInvalidValue;
     const int blocks = f.tiles_x * f.tiles_y * p.n;
+    // 1024 lanes: a tile is one workgroup per CU at these sizes, and its
+    // substeps are LDS-latency bound (256 lanes: 15.6 us at 512^2, two
+    // species, four substeps; 1024 lanes about 5 us less)
+    const int th = 1024;
     switch (f.steps) {
-        case 1: hipLaunchKernelGGL(field_planes_kernel<1>, dim3(blocks), dim3(256), 0, stream, p); break;
-        case 2: hipLaunchKernelGGL(field_planes_kernel<2>, dim3(blocks), dim3(256), 0, stream, p); break;
-        case 3: hipLaunchKernelGGL(field_planes_kernel<3>, dim3(blocks), dim3(256), 0, stream, p); break;
-        case 4: hipLaunchKernelGGL(field_planes_kernel<4>, dim3(blocks), dim3(256), 0, stream, p); break;
+        case 1: hipLaunchKernelGGL(field_planes_kernel<1>, dim3(blocks), dim3(th), 0, stream, p); break;
+        case 2: hipLaunchKernelGGL(field_planes_kernel<2>, dim3(blocks), dim3(th), 0, stream, p); break;
+        case 3: hipLaunchKernelGGL(field_planes_kernel<3>, dim3(blocks), dim3(th), 0, stream, p); break;
+        case 4: hipLaunchKernelGGL(field_planes_kernel<4>, dim3(blocks), dim3(th), 0, stream, p); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

@@ -175,6 +175,7 @@ struct CellArgs {
     double *geo;           // 6 per label
     const int32_t *nematic; // (Q_xx, Q_xy) per site in units of 2^-24, or null
     uint32_t *flags;       // [0] event, [1] recentre, [2] overflow, [3] compacted count
+    uint32_t *flags_next;  // the next step's flag words (cleared by this step's plan), or null
     int64_t *state_copy;   // staging for records moved by a compaction (the other base buffer)
     int64_t *deltas[3];    // the pass delta buffers, cleared for a daughter's fresh id
     const DeviceTables *tables;
@@ -226,7 +227,7 @@ hipError_t launch_recount(const CellArgs &a, size_t n_sites, hipStream_t s);
 hipError_t launch_recount_moments(const CellArgs &a, size_t n_sites, hipStream_t s);
 hipError_t launch_initial_anchors(const CellArgs &a, size_t n_sites, hipStream_t s);
 hipError_t launch_recentre_check(const CellArgs &a, size_t n_sites, hipStream_t s);
-hipError_t launch_population(const CellArgs &a, size_t n_sites, uint32_t *type_copy,
+hipError_t launch_population(const CellArgs &a, int64_t *last_deltas, size_t n_sites, uint32_t *type_copy,
                              int32_t *anchor_copy, hipStream_t s);
 
 }  // namespace ghip

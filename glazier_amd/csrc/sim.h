@@ -144,7 +144,10 @@ struct ghip_sim {
         c.plan = plan;
         c.geo = geo;
         c.nematic = nematic;
-        c.flags = flags;
+        // a lattice that divides or dies alternates two sets of flag words
+        // by step, each step's plan clearing the next step's set
+        c.flags = has_pop ? flags + 4 * (mcs & 1) : flags;
+        c.flags_next = has_pop ? flags + 4 * ((mcs + 1) & 1) : nullptr;
         c.state_copy = A[(pass + 1) % 2];
         for (int i = 0; i < 3; ++i) c.deltas[i] = Dl[i];
         c.tables = tables;

@@ -328,7 +328,10 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &a, const int blk) {
                     after[6] = m[6] + sign * ux * uy;
                     after[7] = m[7] + sign * ux * uz;
                     after[8] = m[8] + sign * uy * uz;
-                    double before = a.len_before[id];
+                    // the label's length in the pass's snapshot: in a plane the
+                    // closed form from the words just gathered, in a volume (an
+                    // iterative eigensolver) once per label by the prepass
+                    double before = D3 ? a.len_before[id] : length_of(m, n, false) - tab->tgt_len[ty];
                     double now = length_of(after, n + sign, three) - tab->tgt_len[ty];
                     dh = dh + (float)(tab->lam_len[ty] * (now * now - before * before));
                 }

@@ -38,6 +38,17 @@ def plain_monolayer(side: int = 128, seed: int = 1, surface: bool = False):
     return model, labels, types
 
 
+def reference_model(side: int, seed: int = 1):
+    """The reference's benchmark model (examples/bench.rs:7-25): contact
+    medium-cell 16, cell-cell 8, V_t 64, lambda_V 2, T 10, Moore, cells of
+    side 8 tiling a side x side lattice."""
+    c = CellType(target_volume=64.0, lambda_volume=2.0)
+    m = Model(width=side, height=side, contact=[0.0, 16.0, 16.0, 8.0], types=[CellType(), c],
+              temperature=10.0, neighbour_order=2, seed=seed)
+    n = side // 8
+    return m, tiled_labels(m, 8, n, n, 1), default_cell_types(n * n + 1)
+
+
 def infection_sheet(side: int = 512, seed: int = 7, infected_fraction: float = 0.05):
     epi = CellType(target_volume=64.0, lambda_volume=2.0, target_length=10.0, lambda_length=1.0)
     inf = CellType(target_volume=64.0, lambda_volume=2.0, target_length=10.0, lambda_length=1.0,
