@@ -94,8 +94,14 @@ GHD float exp2_f32(float x) {
     return p * bits_float((uint32_t)(i + 127) << 23);
 }
 
-// log2(x) in f32 for normal x > 0: x = m 2^e with m in [sqrt(1/2), sqrt(2)),
-// then log2 m = (2 / ln 2) atanh(s), s = (m - 1)/(m + 1), by its odd series.
+// log2(x) in f32 for normal x > 0, by the decomposition of Sun's fdlibm log
+// routines: x = 2^e (1 + f) with 1 + f in [sqrt(1/2), sqrt(2)), s = f/(2 + f),
+// and ln(1 + f) = f - f^2/2 + s (f^2/2 + R(s^2)), R(z) = 2z/3 + 2z^2/5 +
+// 2z^3/7 + 2z^4/9 (the atanh series; its first omitted term is below 1e-8
+// relative for |s| < 0.172). The leading part f - f^2/2 keeps its top 12
+// bits in `hi`, so hi times the 12-bit head of 1/ln 2 is exact and the
+// rounding lands on small terms: within 1.5 ulp of log2 wherever the engine
+// calls it (tested against the C library).
 GHD float log2_f32(float x) {
     if (!(x >= 1.17549435e-38f)) return -__builtin_huge_valf();
     uint32_t b = float_bits(x);
@@ -105,14 +111,52 @@ GHD float log2_f32(float x) {
         m = m * 0.5f;
         e += 1;
     }
-    float s = (m - 1.0f) / (m + 1.0f);
-    float s2 = s * s;
-    float p = 0.11111111f;
-    p = p * s2 + 0.14285714f;
-    p = p * s2 + 0.2f;
-    p = p * s2 + 0.33333333f;
-    p = p * s2 + 1.0f;
-    return (float)e + (s * p) * 2.8853900f;
+    const float f = m - 1.0f;
+    const float s = f / (2.0f + f);
+    const float z = s * s;
+    float r = 0.22222222f;
+    r = r * z + 0.28571429f;
+    r = r * z + 0.4f;
+    r = r * z + 0.66666667f;
+    r = r * z;
+    const float hfsq = 0.5f * f * f;
+    const float hi = bits_float(float_bits(f - hfsq) & 0xfffff000U);
+    const float lo = ((f - hi) - hfsq) + s * (hfsq + r);
+    const float inv_ln2_hi = 1.44287109375f, inv_ln2_lo = -1.7605286e-4f;
+    return (((lo + hi) * inv_ln2_lo + lo * inv_ln2_hi) + hi * inv_ln2_hi) + (float)e;
+}
+
+// The 3 x 3 x 3 cube as 27 bits, bit (dz+1)*9 + (dy+1)*3 + (dx+1): every
+// set position with its 26 neighbours. The Moore (Chebyshev) neighbourhood
+// is the product of the three axes' {-1, 0, 1}, so the dilation is one shift
+// pair per axis, each masked so no bit wraps into the next row or plane.
+GHD uint32_t dilate_cube(uint32_t s) {
+    constexpr uint32_t kCol0 = 0x1249249u;  // dx = -1
+    constexpr uint32_t kCol2 = kCol0 << 2;  // dx = +1
+    constexpr uint32_t kRow0 = 0x01C0E07u;  // dy = -1
+    constexpr uint32_t kRow2 = kRow0 << 6;  // dy = +1
+    constexpr uint32_t kCube = 0x7FFFFFFu;
+    s = (s | ((s << 1) & ~kCol0) | ((s >> 1) & ~kCol2)) & kCube;
+    s = (s | ((s << 3) & ~kRow0) | ((s >> 3) & ~kRow2)) & kCube;
+    return (s | (s << 9) | (s >> 9)) & kCube;
+}
+
+// Whether the members of a volume's 26-position Moore ring (bit k for ring
+// position k in connectivity.rs:23-29's order) form one 26-connected piece
+// (connectivity.rs:43-84): the ring's bits go to the cube's bits around the
+// centre (bit 13), and the piece holding the lowest one is grown by
+// dilation until it stops, the closure of a walk over the ring's adjacency
+// held in registers.
+GHD bool cube_one_piece(uint32_t present) {
+    if (present == 0u || (present & (present - 1u)) == 0u) return true;
+    const uint32_t cube = (present & 0x1FFFu) | ((present & ~0x1FFFu) << 1);
+    uint32_t reached = cube & (~cube + 1u);
+    for (int round = 0; round < 26; ++round) {
+        const uint32_t grown = dilate_cube(reached) & cube;
+        if (grown == reached) break;
+        reached = grown;
+    }
+    return reached == cube;
 }
 
 // Minimum-image offset of an integer coordinate difference, the integer form

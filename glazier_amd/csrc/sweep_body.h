@@ -51,36 +51,13 @@ __device__ __forceinline__ void ring_offset(int k, bool d3, int *dx, int *dy, in
     *dx = q % 3 - 1;
 }
 
-// The 3 x 3 x 3 cube as 27 bits, bit (dz+1)*9 + (dy+1)*3 + (dx+1): every
-// set position with its 26 neighbours. The Moore (Chebyshev) neighbourhood
-// is the product of the three axes' {-1, 0, 1}, so the dilation is one shift
-// pair per axis, each masked so no bit wraps into the next row or plane.
-__device__ __forceinline__ uint32_t dilate_cube(uint32_t s) {
-    constexpr uint32_t kCol0 = 0x1249249u;  // dx = -1
-    constexpr uint32_t kCol2 = kCol0 << 2;  // dx = +1
-    constexpr uint32_t kRow0 = 0x01C0E07u;  // dy = -1
-    constexpr uint32_t kRow2 = kRow0 << 6;  // dy = +1
-    constexpr uint32_t kCube = 0x7FFFFFFu;
-    s = (s | ((s << 1) & ~kCol0) | ((s >> 1) & ~kCol2)) & kCube;
-    s = (s | ((s << 3) & ~kRow0) | ((s >> 3) & ~kRow2)) & kCube;
-    return (s | (s << 9) | (s >> 9)) & kCube;
-}
-
-// Whether the ring positions set in `present` form one piece. In a volume
-// the ring's 26 bits go to the cube's bits around the centre (bit 13) and
-// the piece holding the lowest one is grown until it stops: the same
-// closure as a walk over the ring's adjacency lists, held in registers.
+// Whether the ring positions set in `present` form one piece: the planar
+// ring by the host's 256-bit table, the volume's by cube_one_piece
+// (ghip_common.h).
 __device__ __forceinline__ bool one_piece(uint32_t present, bool d3, const DeviceTables *t) {
     if (present == 0u || (present & (present - 1u)) == 0u) return true;
     if (!d3) return (t->piece2d[present >> 5] >> (present & 31u)) & 1u;
-    const uint32_t cube = (present & 0x1FFFu) | ((present & ~0x1FFFu) << 1);
-    uint32_t reached = cube & (~cube + 1u);
-    for (int round = 0; round < 26; ++round) {
-        const uint32_t grown = dilate_cube(reached) & cube;
-        if (grown == reached) break;
-        reached = grown;
-    }
-    return reached == cube;
+    return cube_one_piece(present);
 }
 
 }  // namespace

@@ -116,6 +116,11 @@ def library() -> ctypes.CDLL:
         "emu_delta_energy": ([v, sz, sz], f64),
         "emu_fields_step": ([v], None),
         "emu_one_piece": ([u32, ctypes.c_int], ctypes.c_int),
+        "emu_activity_mean": ([v, sz, u32], f32),
+        "serial_activity_mean": ([v, sz, u32], f64),
+        "oracle_exp2_f32": ([f32], f32),
+        "oracle_log2_f32": ([f32], f32),
+        "oracle_ring_table": ([ctypes.c_int, u32, u32, P(ctypes.c_uint8)], None),
         "adj_forward": ([ctypes.c_int, ctypes.c_int, ctypes.c_int, u32, P(ctypes.c_int),
                          P(CFieldParams), P(ctypes.c_uint8), u64, P(f64), P(f64)], None),
         "adj_gradient": ([ctypes.c_int, ctypes.c_int, ctypes.c_int, u32, P(ctypes.c_int),
@@ -306,6 +311,10 @@ class SerialSimulation(_Base):
     def locally_connected(self, target: int, label: int) -> bool:
         return bool(library().serial_locally_connected(self._h, target, label))
 
+    def activity_mean(self, site: int, label: int) -> float:
+        """Activity::neighbourhood_mean (motility.rs:56-78) at `site` over `label`."""
+        return float(library().serial_activity_mean(self._h, site, label))
+
     def decay_activity(self) -> None:
         """One Activity::decay (motility.rs:38-45)."""
         library().serial_decay_activity(self._h)
@@ -388,6 +397,11 @@ class EmuSimulation(_Base):
 
     def delta_energy(self, target: int, source: int) -> float:
         return float(library().emu_delta_energy(self._h, target, source))
+
+    def activity_mean(self, site: int, label: int) -> float:
+        """The pricing's f32 geometric mean of the activity over `label` around
+        `site`, on the current lattice."""
+        return float(library().emu_activity_mean(self._h, site, label))
 
     def fields_step(self) -> None:
         library().emu_fields_step(self._h)

@@ -122,8 +122,11 @@ static float g_exp2(float x) {
     return p * bits_to_float((uint32_t)(i + 127) << 23);
 }
 
-/* log2(x) in f32 for normal x > 0: x = m 2^e with m in [sqrt(1/2), sqrt(2)),
- * then log2 m = (2/ln 2) atanh(s), s = (m-1)/(m+1), by its odd series to s^9. */
+/* log2(x) in f32 for normal x > 0 (the engine's log2_f32): x = 2^e (1 + f)
+ * with 1 + f in [sqrt(1/2), sqrt(2)), s = f/(2 + f), and, as Sun's fdlibm
+ * log routines decompose it, ln(1 + f) = f - f^2/2 + s (f^2/2 + R(s^2)) with
+ * R(z) = 2z/3 + 2z^2/5 + 2z^3/7 + 2z^4/9; the head of f - f^2/2 (12 bits)
+ * times the 12-bit head of 1/ln 2 is exact. */
 static float g_log2(float x) {
     if (!(x >= 1.17549435e-38f)) return -INFINITY;
     uint32_t b = float_to_bits(x);
@@ -133,15 +136,25 @@ static float g_log2(float x) {
         m = m * 0.5f;
         e += 1;
     }
-    float s = (m - 1.0f) / (m + 1.0f);
-    float s2 = s * s;
-    float p = 0.11111111f;
-    p = p * s2 + 0.14285714f;
-    p = p * s2 + 0.2f;
-    p = p * s2 + 0.33333333f;
-    p = p * s2 + 1.0f;
-    return (float)e + (s * p) * 2.8853900f;
+    const float f = m - 1.0f;
+    const float s = f / (2.0f + f);
+    const float z = s * s;
+    float r = 0.22222222f;
+    r = r * z + 0.28571429f;
+    r = r * z + 0.4f;
+    r = r * z + 0.66666667f;
+    r = r * z;
+    const float hfsq = 0.5f * f * f;
+    const float hi = bits_to_float(float_to_bits(f - hfsq) & 0xfffff000U);
+    const float lo = ((f - hi) - hfsq) + s * (hfsq + r);
+    const float inv_ln2_hi = 1.44287109375f, inv_ln2_lo = -1.7605286e-4f;
+    return (((lo + hi) * inv_ln2_lo + lo * inv_ln2_hi) + hi * inv_ln2_hi) + (float)e;
 }
+
+/* The two functions as the oracle's callers see them, for the tests that
+ * hold them to the C library. */
+float oracle_exp2_f32(float x) { return g_exp2(x); }
+float oracle_log2_f32(float x) { return g_log2(x); }
 
 /* ---------------------------------------------------------------- state */
 
@@ -653,6 +666,14 @@ static float geo_mean(const emu_sim *e, size_t site, uint32_t label) {
         count = count + 1.0f;
     }
     return g_exp2(lsum / count);
+}
+
+/* The f32 geometric mean of the pricing, over the current lattice and
+ * activity (a pass reads them as they stood at its start). */
+float emu_activity_mean(emu_sim *e, size_t site, uint32_t label) {
+    memcpy(e->prev_labels, e->labels, e->n * 4);
+    memcpy(e->prev_act, e->act, e->n * 4);
+    return geo_mean(e, site, label);
 }
 
 /* Change in the length energy of `label` when the site (x, y, z) joins

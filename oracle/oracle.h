@@ -85,6 +85,12 @@ void oracle_principal_axis(const double c[9], double axis[3], double *lambda);
 typedef struct emu_sim emu_sim;
 
 uint32_t oracle_hash32(uint32_t x);
+/* The emulation's f32 2^x and log2 (the engine's exp2_f32 / log2_f32). */
+float oracle_exp2_f32(float x);
+float oracle_log2_f32(float x);
+/* One byte per ring mask in [lo, hi): bit 0 union-find, bit 1 the
+ * emulation's breadth-first one-piece test (ringcheck.c). */
+void oracle_ring_table(int ring_size, uint32_t lo, uint32_t hi, uint8_t *out);
 uint32_t oracle_pass_key(uint64_t seed, uint32_t replica, uint64_t step, uint32_t colour);
 uint32_t oracle_site_pick(uint32_t key, uint32_t site);
 uint32_t oracle_site_draw(uint32_t key, uint32_t site);

@@ -581,6 +581,10 @@ static double neighbourhood_mean(const serial_sim *s, size_t site, uint32_t labe
     return exp(log_sum / count);
 }
 
+double serial_activity_mean(const serial_sim *s, size_t site, uint32_t label) {
+    return neighbourhood_mean(s, site, label);
+}
+
 static double step_fold(int64_t a, int64_t b, double span) {
     double raw = (double)(a - b);
     if (raw > span / 2.0) return raw - span;

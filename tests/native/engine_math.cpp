@@ -1,0 +1,18 @@
+// Host build of arithmetic the kernels share with the host
+// (glazier_amd/csrc/ghip_common.h): the engine's own f32 exponential and
+// logarithm, and the volume's connectivity test, so the CPU tests can hold
+// them to the C library and to the oracle.
+#include "../../glazier_amd/csrc/ghip_common.h"
+
+extern "C" {
+void engine_exp2_f32(const float *x, float *y, long n) {
+    for (long i = 0; i < n; ++i) y[i] = ghip::exp2_f32(x[i]);
+}
+void engine_log2_f32(const float *x, float *y, long n) {
+    for (long i = 0; i < n; ++i) y[i] = ghip::log2_f32(x[i]);
+}
+// One byte per ring mask in [lo, hi): whether its members are one piece.
+void engine_ring_table(uint32_t lo, uint32_t hi, uint8_t *out) {
+    for (uint32_t m = lo; m < hi; ++m) out[m - lo] = ghip::cube_one_piece(m) ? 1 : 0;
+}
+}
