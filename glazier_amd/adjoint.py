@@ -39,6 +39,7 @@ def _bind():
     lib.ghip_fit_model_params.argtypes = [v, P(CFieldParams)]
     lib.ghip_fit_forward.argtypes = [v, P(CFieldParams), v, v, v]
     lib.ghip_fit_backward.argtypes = [v, v, P(CFieldParams), v, v]
+    lib.ghip_fit_backward_noalloc.argtypes = [v, v, P(CFieldParams), v, v]
     _bound = True
     return lib
 

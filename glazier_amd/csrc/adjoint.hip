@@ -25,6 +25,8 @@
 // collects block b's share of every step, in step order, with no atomics and
 // no contention); one reduction over the slots ends the backward pass. The
 // gradient is therefore the same on every run.
+#include <mutex>
+
 #include "fit.h"
 
 namespace ghip {
@@ -609,6 +611,48 @@ hipError_t launch_fit_reduce(uint32_t species, uint32_t blocks, const double *pa
                              hipStream_t st) {
     hipLaunchKernelGGL(fit_reduce_kernel, dim3(kPartWords, species), dim3(256), 0, st, part, blocks, grad);
     return hipGetLastError();
+}
+
+__global__ void fit_zero_kernel(double *p, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        p[i] = 0.0;
+}
+
+hipError_t launch_fit_zero(double *p, size_t n, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    size_t blocks = (n + 255) / 256;
+    if (blocks > 1024) blocks = 1024;
+    hipLaunchKernelGGL(fit_zero_kernel, dim3((unsigned)blocks), dim3(256), 0, st, p, n);
+    return hipGetLastError();
+}
+
+namespace {
+
+// Scratch of ghip_fit_backward_noalloc (partial sums and their reduction).
+// Each GPU holds its own copy until the process ends; host code never names
+// it directly, only the per-device address fit_scratch looked up.
+static __device__ __align__(256) double scratch[kFitScratchWords];
+
+constexpr int kScratchDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_at[kScratchDevices] = {};
+
+}  // namespace
+
+hipError_t fit_scratch(double **out) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= kScratchDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> hold(g_scratch_mu);
+    if (!g_scratch_at[dev]) {
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_at[dev] = static_cast<double *>(p);
+    }
+    *out = g_scratch_at[dev];
+    return hipSuccess;
 }
 
 }  // namespace ghip

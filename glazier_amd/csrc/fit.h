@@ -58,5 +58,14 @@ uint32_t fit_blocks(const FitGrid &g);
 uint32_t fit_tiles(const FitGrid &g);
 hipError_t launch_fit_reduce(uint32_t species, uint32_t blocks, const double *part, double *grad,
                              hipStream_t st);
+// Zeroes n doubles (a kernel, so a captured stream holds no memset node).
+hipError_t launch_fit_zero(double *p, size_t n, hipStream_t st);
+
+// The library's static scratch for a backward pass that allocates nothing:
+// kFitScratchWords doubles (4 MiB) per device, holding the partial sums and
+// their reduction. fit_scratch gives the current device's address, looked
+// up once per device.
+constexpr size_t kFitScratchWords = (size_t)1 << 19;
+hipError_t fit_scratch(double **out);
 
 }  // namespace ghip

@@ -246,13 +246,17 @@ ghip_status ghip_fit_forward(ghip_fit *f, const ghip_field_params *p, const floa
     return GHIP_OK;
 }
 
-ghip_status ghip_fit_backward(ghip_fit *f, const float *grad_final, ghip_field_params *grad, float *grad_u0,
-                              void *stream) {
-    if (!f || !grad_final || !grad || !grad_u0) return fail(GHIP_BAD_ARG, "null argument");
-    if (!f->have_forward) return fail(GHIP_BAD_ARG, "backward needs a forward pass first");
-    hipStream_t st = stream ? (hipStream_t)stream : f->own;
+}  // extern "C"
+
+namespace {
+
+// The backward pass with the partial sums in `part` (S * part_blocks *
+// kPartWords doubles, cleared here) and their reduction in `red` (S *
+// kPartWords); the host copy of the reduction stays on the stack.
+ghip_status backward_into(ghip_fit *f, const float *grad_final, ghip_field_params *grad, float *grad_u0,
+                          hipStream_t st, double *part, double *red) {
     const size_t fw = f->field_words();
-    HIP_TRY(hipMemsetAsync(f->part, 0, (size_t)f->S * f->part_blocks() * kPartWords * sizeof(double), st));
+    HIP_TRY(launch_fit_zero(part, (size_t)f->S * f->part_blocks() * kPartWords, st));
     HIP_TRY(hipMemcpyAsync(f->lam[0], grad_final, fw * 4, hipMemcpyDeviceToDevice, st));
     int li = 0;
     const uint64_t n_seg = (f->steps + f->K - 1) / f->K;
@@ -270,7 +274,7 @@ ghip_status ghip_fit_backward(ghip_fit *f, const float *grad_final, ghip_field_p
             if (f->fused) {
                 HIP_TRY(launch_fit_step_adjoint_fused(f->g, f->coef, f->S, f->max_sub, f->types + m * f->g.n,
                                                       j == 0 ? start0 : f->seg + j * fw, f->lam[li],
-                                                      f->lam[li ^ 1], f->part, st));
+                                                      f->lam[li ^ 1], part, st));
                 li ^= 1;
                 continue;
             }
@@ -282,23 +286,23 @@ ghip_status ghip_fit_backward(ghip_fit *f, const float *grad_final, ghip_field_p
             }
             const float *pre = f->sub + (size_t)(f->max_sub - 1) * fw;
             HIP_TRY(launch_fit_exchange_adjoint(f->g, f->coef, f->S, f->types + m * f->g.n, pre, f->lam[li],
-                                                f->part, st));
+                                                part, st));
             for (int s = f->max_sub; s >= 1; --s) {
                 const float *before = s == 1 ? start : f->sub + (size_t)(s - 2) * fw;
                 HIP_TRY(launch_fit_substep_adjoint(f->g, f->coef, f->S, f->lam[li], before, f->lam[li ^ 1],
-                                                   f->part, s, st));
+                                                   part, s, st));
                 li ^= 1;
             }
         }
     }
     HIP_TRY(hipMemcpyAsync(grad_u0, f->lam[li], fw * 4, hipMemcpyDeviceToDevice, st));
-    HIP_TRY(launch_fit_reduce(f->S, f->part_blocks(), f->part, f->grad, st));
-    std::vector<double> acc((size_t)f->S * kPartWords);
-    HIP_TRY(hipMemcpyAsync(acc.data(), f->grad, acc.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIP_TRY(launch_fit_reduce(f->S, f->part_blocks(), part, red, st));
+    double acc[kMaxSpecies * kPartWords];
+    HIP_TRY(hipMemcpyAsync(acc, red, (size_t)f->S * kPartWords * sizeof(double), hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
     std::memset(grad, 0, sizeof *grad);
     for (uint32_t k = 0; k < f->S; ++k) {
-        const double *w = acc.data() + (size_t)k * kPartWords;
+        const double *w = acc + (size_t)k * kPartWords;
         const double sub = (double)f->m.species[k].substeps;
         grad->diffusion[k] = w[0] / sub;
         grad->decay[k] = w[1] / sub;
@@ -308,6 +312,34 @@ ghip_status ghip_fit_backward(ghip_fit *f, const float *grad_final, ghip_field_p
         }
     }
     return GHIP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+ghip_status ghip_fit_backward(ghip_fit *f, const float *grad_final, ghip_field_params *grad, float *grad_u0,
+                              void *stream) {
+    if (!f || !grad_final || !grad || !grad_u0) return fail(GHIP_BAD_ARG, "null argument");
+    if (!f->have_forward) return fail(GHIP_BAD_ARG, "backward needs a forward pass first");
+    hipStream_t st = stream ? (hipStream_t)stream : f->own;
+    return backward_into(f, grad_final, grad, grad_u0, st, f->part, f->grad);
+}
+
+ghip_status ghip_fit_backward_noalloc(ghip_fit *f, const float *grad_final, ghip_field_params *grad,
+                                      float *grad_u0, void *stream) {
+    if (!f || !grad_final || !grad || !grad_u0) return fail(GHIP_BAD_ARG, "null argument");
+    if (!f->have_forward) return fail(GHIP_BAD_ARG, "backward needs a forward pass first");
+    const size_t part_words = (size_t)f->S * f->part_blocks() * kPartWords;
+    const size_t need = part_words + (size_t)f->S * kPartWords;
+    if (need > kFitScratchWords)
+        return fail(GHIP_UNSUPPORTED, "this fit's backward pass needs " + std::to_string(need) +
+                                          " words of scratch; the static scratch holds " +
+                                          std::to_string(kFitScratchWords) + " (use ghip_fit_backward)");
+    double *base = nullptr;
+    HIP_TRY(fit_scratch(&base));
+    hipStream_t st = stream ? (hipStream_t)stream : f->own;
+    return backward_into(f, grad_final, grad, grad_u0, st, base, base + part_words);
 }
 
 }  // extern "C"

@@ -274,6 +274,21 @@ ghip_status ghip_fit_forward(ghip_fit *fit, const ghip_field_params *params, con
 ghip_status ghip_fit_backward(ghip_fit *fit, const float *grad_final, ghip_field_params *grad,
                               float *grad_u0, void *stream);
 
+/* ghip_fit_backward that allocates nothing. No counterpart in the reference:
+ * it is for callers that cannot allocate (code under HIP graph capture,
+ * serving loops, harnesses that forbid hipMalloc). Same arguments, results
+ * and errors as ghip_fit_backward, bit for bit; the kernels' partial sums and
+ * their reduction live in the library's static device scratch (4 MiB per
+ * GPU, held until the process ends) instead of the fit's buffers, and are
+ * cleared by a kernel, not a memset. A fit whose pass needs more than the
+ * scratch holds (species x (workgroups + 1) x 34 doubles: a two-species
+ * plane of up to about 3.9 million sites, a volume of about 2 million)
+ * fails with GHIP_UNSUPPORTED. Every call on
+ * one GPU shares that GPU's scratch: calls that overlap on one GPU (other
+ * streams or host threads) race and must be serialised by the caller. */
+ghip_status ghip_fit_backward_noalloc(ghip_fit *fit, const float *grad_final, ghip_field_params *grad,
+                                      float *grad_u0, void *stream);
+
 /* Message of the last failure on this thread; empty when none. */
 const char *ghip_last_error(void);
 
