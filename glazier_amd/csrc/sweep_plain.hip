@@ -45,29 +45,37 @@ namespace ghip {
 constexpr int kPThr = 256;  // acceptance-table entries held in LDS
 
 namespace {
-// Count of neighbours differing from v: sum of min(n ^ v, 1).
+// Count of neighbours differing from v: each compare's bit added as a
+// carry (a compare and an add-with-carry per neighbour).
 __device__ __forceinline__ uint32_t unequal8(uint32_t v, uint32_t n0, uint32_t n1, uint32_t n2,
                                              uint32_t n3, uint32_t n4, uint32_t n5, uint32_t n6,
                                              uint32_t n7) {
-    const uint32_t a = min(n0 ^ v, 1u) + min(n1 ^ v, 1u) + min(n2 ^ v, 1u);
-    const uint32_t b = min(n3 ^ v, 1u) + min(n4 ^ v, 1u) + min(n5 ^ v, 1u);
-    return a + b + min(n6 ^ v, 1u) + min(n7 ^ v, 1u);
+    uint32_t c = (uint32_t)(n0 != v);
+    c += (uint32_t)(n1 != v);
+    c += (uint32_t)(n2 != v);
+    c += (uint32_t)(n3 != v);
+    c += (uint32_t)(n4 != v);
+    c += (uint32_t)(n5 != v);
+    c += (uint32_t)(n6 != v);
+    c += (uint32_t)(n7 != v);
+    return c;
 }
 
 // The decision on one target whose label would change from `old` to `nw`:
 // contact gained minus lost from neighbour counts (exact with integer
 // energies), volume and surface against each label's pass-start record
 // (volume * 2^32 + surface), then the integral acceptance table. The
-// uniform's hash round (of the target's pick round `ph`) is taken only when
-// the table decides, 0 < dH < its length. Shared by the streaming pass and
-// the resident one, so both decide identically.
+// target's two hash rounds (the pick round of `site`, then the uniform's)
+// are taken only when the table decides, 0 < dH < its length. Shared by the
+// streaming pass and the resident one, so both decide identically.
 struct PlainMove {
     bool acc;
     int ds_old, ds_new;
 };
 
-__device__ __forceinline__ PlainMove plain_move(const PlainParams &p, const uint32_t *thr_s, uint32_t key2,
-                                                uint32_t ph, uint32_t old, uint32_t nw, uint32_t n0, uint32_t n1,
+__device__ __forceinline__ PlainMove plain_move(const PlainParams &p, const uint32_t *thr_s, uint32_t key,
+                                                uint32_t key2, uint32_t site, uint32_t old, uint32_t nw,
+                                                uint32_t n0, uint32_t n1,
                                                 uint32_t n2, uint32_t n3, uint32_t n4, uint32_t n5,
                                                 uint32_t n6, uint32_t n7, int64_t pv_old, int64_t pv_new,
                                                 bool has_medium) {
@@ -117,7 +125,7 @@ __device__ __forceinline__ PlainMove plain_move(const PlainParams &p, const uint
         dh = dh + p.lam_s * (dsf * (2.0f * (s - p.tgt_s) + dsf));
     }
     bool acc = dh <= 0.0f;
-    if (!acc && dh < (float)p.n_thr) acc = (site_draw(key2, ph) >> 8) < thr_s[(int)dh];
+    if (!acc && dh < (float)p.n_thr) acc = (site_draw(key2, site_pick(key, site)) >> 8) < thr_s[(int)dh];
     mv.acc = !emptying && acc;
     return mv;
 }
@@ -135,7 +143,10 @@ struct PlainTile {
     static constexpr int kTargets = PX * PY / 4;
     static constexpr int kTRow = PX / 2;  // targets per target row
     static_assert(kTRow % 64 == 0, "a wavefront's targets share one row");
-    static_assert(kTargets <= 65536, "16-bit queue entries");
+    // a queue entry is the target's index in the tile with its pick (the
+    // neighbour, 0..7) above it, so pricing takes no second hash round
+    static constexpr int kQBits = kTargets == 2048 ? 11 : kTargets == 1024 ? 10 : kTargets == 512 ? 9 : 8;
+    static_assert((1 << kQBits) == kTargets && kQBits + 3 <= 16, "16-bit queue entries");
 
     const uint32_t *tile;
     int x0, y0;
@@ -146,45 +157,45 @@ struct PlainTile {
         *site = (uint32_t)(y0 + 2 * j + a.cy) * (uint32_t)a.w + (uint32_t)(x0 + a.cx + 2 * i);
         *tl = (1 + 2 * j + a.cy) * kPitch + 4 + a.cx + 2 * i;
     }
-    // the neighbour a target's pick round `ph` names: k = ph & 7 in ORDER3's
-    // planar order, (dx, dy) row by row; dx + 1 and dy + 1 are the k-th 2-bit
-    // fields of 0x9224 and 0xA940
-    __device__ __forceinline__ uint32_t picked(uint32_t ph, int tl) const {
-        const uint32_t sh = (ph & 7u) * 2u;
+    // the neighbour pick k (0..7) names, in ORDER3's planar order, (dx, dy)
+    // row by row; dx + 1 and dy + 1 are the k-th 2-bit fields of 0x9224 and
+    // 0xA940
+    __device__ __forceinline__ uint32_t picked(uint32_t k, int tl) const {
+        const uint32_t sh = k * 2u;
         const int dxy = (int)(((0xA940u >> sh) & 3u) * (uint32_t)kPitch + ((0x9224u >> sh) & 3u));
         return tile[tl - kPitch - 1 + dxy];
     }
 
-    // Phase A: draw every target, queue the ones whose label would change.
-    // A lane's targets are q = k NT + tid; bit k of `moves` says target k
-    // would change. The wavefront reserves queue space once for all of them:
-    // a lane's offset is the prefix sum of the lanes' counts, taken from the
-    // ballots of the count's three bits.
-    template <int NT>
-    __device__ __forceinline__ void phase_a(const SweepArgs &a, uint16_t *queue, uint32_t *qcount) const {
-        queue_moves<NT>(draw<NT, 0, kTargets / NT>(a), queue, qcount);
-    }
-
-    // The draws of a lane's targets k in [K0, K1): bit k set when target k
-    // would change. Target k of the lane lies on tile row 2 j + cy, j =
-    // (k NT + tid) / kTRow, and reads the LDS rows either side of it.
+    // Phase A draws every target and queues the ones whose label would
+    // change. A lane's targets are q = k NT + tid; bit k of `moves` says
+    // target k would change, and bits 3k..3k+2 of `picks` hold its pick.
+    // Target k of the lane lies on tile row 2 j + cy, j = (k NT + tid) /
+    // kTRow, and reads the LDS rows either side of it; NT is a multiple of
+    // kTRow, so the lane's column is fixed and its rows step by NT / kTRow
+    // from one k to the next.
     template <int NT, int K0, int K1>
-    __device__ __forceinline__ uint32_t draw(const SweepArgs &a) const {
-        uint32_t moves = 0;
+    __device__ __forceinline__ void draw(const SweepArgs &a, uint32_t &moves, uint32_t &picks) const {
+        static_assert(NT % kTRow == 0, "a lane keeps its column");
+        constexpr int kJStep = NT / kTRow;
+        const int jl = (int)threadIdx.x / kTRow, il = (int)threadIdx.x % kTRow;
+        const uint32_t site0 = (uint32_t)(y0 + 2 * jl + a.cy) * (uint32_t)a.w + (uint32_t)(x0 + a.cx + 2 * il);
+        const int tl0 = (1 + 2 * jl + a.cy) * kPitch + 4 + a.cx + 2 * il;
+        const uint32_t dsite = 2u * (uint32_t)kJStep * (uint32_t)a.w;
 #pragma unroll
         for (int k = K0; k < K1; ++k) {
-            uint32_t site;
-            int tl;
-            target(a, k * NT + (int)threadIdx.x, &site, &tl);
-            moves |= (tile[tl] != picked(site_pick(a.key, site), tl) ? 1u : 0u) << k;
+            const uint32_t site = site0 + (uint32_t)k * dsite;
+            const int tl = tl0 + k * 2 * kJStep * kPitch;
+            const uint32_t pk = site_pick(a.key, site) & 7u;
+            moves |= (tile[tl] != picked(pk, tl) ? 1u : 0u) << k;
+            picks |= pk << (3 * k);
         }
-        return moves;
     }
 
-    // The queue entries of the targets `moves` marks: one reservation per
-    // wavefront.
+    // The queue entries of the targets `moves` marks, each with its pick:
+    // one reservation per wavefront.
     template <int NT>
-    __device__ __forceinline__ void queue_moves(uint32_t moves, uint16_t *queue, uint32_t *qcount) const {
+    __device__ __forceinline__ void queue_moves(uint32_t moves, uint32_t picks, uint16_t *queue,
+                                                uint32_t *qcount) const {
         constexpr int kPerLane = kTargets / NT;
         static_assert(kTargets % NT == 0 && kPerLane <= 7, "a lane's count fits three bits");
         const int tid = threadIdx.x;
@@ -202,7 +213,8 @@ struct PlainTile {
         uint32_t pos = base + below(c0) + 2u * below(c1) + 4u * below(c2);
 #pragma unroll
         for (int k = 0; k < kPerLane; ++k) {
-            if (moves & (1u << k)) queue[pos++] = (uint16_t)(k * NT + tid);
+            if (moves & (1u << k))
+                queue[pos++] = (uint16_t)((k * NT + tid) | (int)((picks >> (3 * k)) & 7u) << kQBits);
         }
     }
 
@@ -228,10 +240,10 @@ struct PlainTile {
         for (uint32_t qi = threadIdx.x; qi < nq; qi += NT) {
             uint32_t site;
             int tl;
-            target(a, queue[qi], &site, &tl);
-            const uint32_t ph = site_pick(a.key, site);
+            const uint32_t entry = queue[qi];
+            target(a, (int)(entry & (uint32_t)(kTargets - 1)), &site, &tl);
             const uint32_t old = tile[tl];
-            const uint32_t nw = picked(ph, tl);
+            const uint32_t nw = picked(entry >> kQBits, tl);
             const uint32_t oid = label_id(old), nid = label_id(nw);
             // the gathers go out together, before anything waits on them;
             // label 0's record is real memory whose value is never used. When
@@ -246,8 +258,8 @@ struct PlainTile {
             const uint32_t n3 = tile[tl - 1], n4 = tile[tl + 1];
             const uint32_t n5 = tile[tl + kPitch - 1], n6 = tile[tl + kPitch], n7 = tile[tl + kPitch + 1];
             const PlainMove mv =
-                plain_move(p, thr_s, a.key2, ph, old, nw, n0, n1, n2, n3, n4, n5, n6, n7, ao + eo, an + en,
-                           has_medium);
+                plain_move(p, thr_s, a.key, a.key2, site, old, nw, n0, n1, n2, n3, n4, n5, n6, n7, ao + eo,
+                           an + en, has_medium);
             if (!mv.acc) continue;
             a.labels[site] = nw;
             if (a.merge_deltas && qi < (uint32_t)NT) {
@@ -375,17 +387,25 @@ __global__ __launch_bounds__(NT) void sweep_plain2d_kernel(SweepArgs a, PlainPar
     constexpr int kSpare = kRows * kPitch + 1;
     const int hpos = h_live ? (tid >> 1) * kPitch + ((tid & 1) ? 4 + PX : 3) : kSpare + (tid & 1);
     const uint32_t tv = p.thr[tid < (int)p.n_thr ? tid : 0];
+    // load k of a lane reads tile row r = k kRowsPerLoad + tid / kC4 (row 0
+    // is the halo row above, kRows - 1 the one below, each wrapping at the
+    // lattice's edge; rows past the tile reread the last one), column 4 (tid
+    // % kC4): one multiply per lane, the rest steps of a row offset
+    constexpr int kRowsPerLoad = NT / kC4;
+    static_assert(NT % kC4 == 0, "a load instruction covers whole rows");
+    const int c4 = tid % kC4, rl = tid / kC4;
+    const uint32_t *col = a.labels + (uint32_t)x0 + 4u * (uint32_t)c4;
+    const uint32_t off_top = (uint32_t)(y0 == 0 ? H - 1 : y0 - 1) * (uint32_t)W;
+    const uint32_t off_bottom = (uint32_t)(y0 + PY >= H ? 0 : y0 + PY) * (uint32_t)W;
+    const uint32_t off_lane = (uint32_t)(y0 - 1 + rl) * (uint32_t)W;
+    const uint32_t row_step = (uint32_t)kRowsPerLoad * (uint32_t)W;
     uint4 v[kLoads];
     int vdst[kLoads];
 #pragma unroll
     for (int k = 0; k < kLoads; ++k) {
-        const int idx = k * NT + tid;
-        const int r = idx / kC4, c4 = idx % kC4;
-        int g = y0 - 1 + (r < kRows ? r : kRows - 1);
-        g += g < 0 ? H : 0;
-        g -= g >= H ? H : 0;
-        v[k] = *reinterpret_cast<const uint4 *>(a.labels + (uint32_t)g * (uint32_t)W + (uint32_t)x0 +
-                                                4u * (uint32_t)c4);
+        const int r = k * kRowsPerLoad + rl;
+        const uint32_t off = r == 0 ? off_top : r >= kRows - 1 ? off_bottom : off_lane + (uint32_t)k * row_step;
+        v[k] = *reinterpret_cast<const uint4 *>(col + off);
         vdst[k] = r < kRows ? r * kPitch + 4 + 4 * c4 : -1;
     }
     const uint32_t flo = blockIdx.x * a.fold_per_block;
@@ -404,8 +424,6 @@ __global__ __launch_bounds__(NT) void sweep_plain2d_kernel(SweepArgs a, PlainPar
     // ---- land them in two parts. The first kL1 loads hold the tile's first
     // rows; the lane's first kA1 targets read no row beyond them, so they are
     // drawn while the rest of the tile is still arriving.
-    constexpr int kRowsPerLoad = NT / kC4;
-    static_assert(NT % kC4 == 0, "a load instruction covers whole rows");
     constexpr int kL1 = (kLoads + 1) / 2;
     constexpr int kR1 = kL1 * kRowsPerLoad < kRows ? kL1 * kRowsPerLoad : kRows;  // rows in the first part
     constexpr int kTRowsPerDraw = NT / T::kTRow;  // target rows one draw of every lane covers
@@ -426,7 +444,7 @@ __global__ __launch_bounds__(NT) void sweep_plain2d_kernel(SweepArgs a, PlainPar
         }
     };
     const PlainTile<PX, PY> pt{tile, x0, y0};
-    uint32_t moves = 0;
+    uint32_t moves = 0, picks = 0;
     land(0, kA1 > 0 ? kL1 : kLoads);
     tile[hpos] = hv;
     lo = min(lo, h_live ? hv : 0xFFFFFFFFu);
@@ -435,7 +453,7 @@ __global__ __launch_bounds__(NT) void sweep_plain2d_kernel(SweepArgs a, PlainPar
     *(tid < kPThr ? &thr_s[tid] : &tile[kSpare + (tid & 1)]) = tv;
     if constexpr (kA1 > 0) {
         __syncthreads();
-        moves = pt.template draw<NT, 0, kA1>(a);
+        pt.template draw<NT, 0, kA1>(a, moves, picks);
         land(kL1, kLoads);
     }
     const bool e_zero = a.pass_flags && f1 == 0;
@@ -452,8 +470,8 @@ __global__ __launch_bounds__(NT) void sweep_plain2d_kernel(SweepArgs a, PlainPar
     }
     const bool has_medium = __syncthreads_or(p.needs_medium && lo == 0u) != 0;
 
-    moves |= pt.template draw<NT, kA1, kPerLane>(a);
-    pt.template queue_moves<NT>(moves, queue, &qcount);
+    pt.template draw<NT, kA1, kPerLane>(a, moves, picks);
+    pt.template queue_moves<NT>(moves, picks, queue, &qcount);
     __syncthreads();
     uint32_t d_old, d_new;
     const uint32_t my_acc = pt.template phase_b<NT>(a, p, thr_s, queue, qcount, has_medium, e_zero, d_old, d_new);
@@ -575,7 +593,7 @@ __global__ __launch_bounds__(kResThreads) void sweep_plain2d_resident_kernel(Pla
                 const uint32_t nw = k == 0 ? n0 : k == 1 ? n1 : k == 2 ? n2 : k == 3 ? n3
                                   : k == 4 ? n4 : k == 5 ? n5 : k == 6 ? n6 : n7;
                 const uint32_t oid = label_id(old), nid = label_id(nw);
-                const PlainMove mv = plain_move(p, thr_s, key2, ph, old, nw, n0, n1, n2, n3, n4, n5, n6, n7,
+                const PlainMove mv = plain_move(p, thr_s, key, key2, site, old, nw, n0, n1, n2, n3, n4, n5, n6, n7,
                                                 rec[oid], rec[nid], p.needs_medium != 0);
                 if (!mv.acc) continue;
                 lat[site] = nw;
