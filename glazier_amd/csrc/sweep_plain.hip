@@ -73,15 +73,27 @@ struct PlainMove {
     int ds_old, ds_new;
 };
 
-__device__ __forceinline__ PlainMove plain_move(const PlainParams &p, const uint32_t *thr_s, uint32_t key,
-                                                uint32_t key2, uint32_t site, uint32_t old, uint32_t nw,
-                                                uint32_t n0, uint32_t n1,
-                                                uint32_t n2, uint32_t n3, uint32_t n4, uint32_t n5,
-                                                uint32_t n6, uint32_t n7, int64_t pv_old, int64_t pv_new,
-                                                bool has_medium) {
+// The neighbourhood's part of a decision: neighbours unlike the old label,
+// unlike the new one, and (when the medium can change a price) medium ones.
+struct PlainCounts {
+    int unlike_old, unlike_new, med;
+};
+
+__device__ __forceinline__ PlainCounts plain_counts(uint32_t old, uint32_t nw, uint32_t n0, uint32_t n1,
+                                                    uint32_t n2, uint32_t n3, uint32_t n4, uint32_t n5,
+                                                    uint32_t n6, uint32_t n7, bool has_medium) {
+    PlainCounts c;
+    c.unlike_old = (int)unequal8(old, n0, n1, n2, n3, n4, n5, n6, n7);
+    c.unlike_new = (int)unequal8(nw, n0, n1, n2, n3, n4, n5, n6, n7);
+    c.med = has_medium ? 8 - (int)unequal8(0u, n0, n1, n2, n3, n4, n5, n6, n7) : 0;
+    return c;
+}
+
+__device__ __forceinline__ PlainMove plain_move_counts(const PlainParams &p, const uint32_t *thr_s, uint32_t key,
+                                                       uint32_t key2, uint32_t site, uint32_t old, uint32_t nw,
+                                                       const PlainCounts &c, int64_t pv_old, int64_t pv_new) {
     const uint32_t oid = label_id(old), nid = label_id(nw);
-    const int unlike_old = (int)unequal8(old, n0, n1, n2, n3, n4, n5, n6, n7);
-    const int unlike_new = (int)unequal8(nw, n0, n1, n2, n3, n4, n5, n6, n7);
+    const int unlike_old = c.unlike_old, unlike_new = c.unlike_new;
     const int like_old = 8 - unlike_old, like_new = 8 - unlike_new;
     // absolute states keep surface in [0, 2^31): volume is the high word
     const int vol_old = (int)(uint32_t)((uint64_t)pv_old >> 32);
@@ -89,8 +101,7 @@ __device__ __forceinline__ PlainMove plain_move(const PlainParams &p, const uint
     // a copy that would empty a cell is refused (no branch before the
     // records are used, so no gather waits behind another)
     const bool emptying = oid != 0 && vol_old <= 1;
-    int med = 0;
-    if (has_medium) med = 8 - (int)unequal8(0u, n0, n1, n2, n3, n4, n5, n6, n7);
+    const int med = c.med;
     const int etn = nid != 0 ? 1 : 0, eto = oid != 0 ? 1 : 0;
     // neighbours unlike the new label: 8 - like_new; the medium ones among
     // them are all medium neighbours when the new label is a cell, none
@@ -128,6 +139,15 @@ __device__ __forceinline__ PlainMove plain_move(const PlainParams &p, const uint
     if (!acc && dh < (float)p.n_thr) acc = (site_draw(key2, site_pick(key, site)) >> 8) < thr_s[(int)dh];
     mv.acc = !emptying && acc;
     return mv;
+}
+
+__device__ __forceinline__ PlainMove plain_move(const PlainParams &p, const uint32_t *thr_s, uint32_t key,
+                                                uint32_t key2, uint32_t site, uint32_t old, uint32_t nw,
+                                                uint32_t n0, uint32_t n1, uint32_t n2, uint32_t n3, uint32_t n4,
+                                                uint32_t n5, uint32_t n6, uint32_t n7, int64_t pv_old,
+                                                int64_t pv_new, bool has_medium) {
+    const PlainCounts c = plain_counts(old, nw, n0, n1, n2, n3, n4, n5, n6, n7, has_medium);
+    return plain_move_counts(p, thr_s, key, key2, site, old, nw, c, pv_old, pv_new);
 }
 }  // namespace
 
