@@ -225,7 +225,7 @@ __global__ __launch_bounds__(THREADS) void batch_cell_kernel(BatchArgs b, WorkLi
     } else if constexpr (OP == kOpRecentre1) {
         recentre_body(c, 1, lb, gb);
     } else if constexpr (OP == kOpRecountMoments1) {
-        recount_body(c, n, 1024, 1, 1, lb, gb);
+        recount_body(c, n, 1, 1, lb, gb);
     } else if constexpr (OP == kOpDecay) {
         // activity_decay_kernel (fields.hip) on this replica's buffer
         float *act = b.cur_act[r];

@@ -67,7 +67,7 @@ uint32_t cell_blocks(const ghip_sim *s, int op) {
         case kOpDecay:
             return cdiv(s->N, 256);
         case kOpRecountMoments1:
-            return cdiv(s->N, 1024);
+            return recount_blocks(s->W, s->H, s->D);
         default:
             return cdiv(s->cap, 256);
     }

@@ -79,15 +79,21 @@ __device__ __forceinline__ void anchor_from_first_body(const CellArgs &a, const 
 }
 
 // Recount of every label from the lattice into a zeroed state (or, with
-// moments_only, of the moment words alone). A workgroup pre-aggregates its
-// sites per label in an LDS hash table with LDS atomics, then flushes one
-// global atomic per (label, word); labels that find no free slot go straight
-// to global atomics. `gate` >= 0 makes the launch a no-op unless
-// flags[gate] is set, so a whole step can be enqueued without host checks.
+// moments_only, of the moment words alone). A workgroup covers kRecountRun
+// lattice rows (rows of the flattened (z, y) index) of 256 columns; each lane
+// walks its column down those rows, so every load of the workgroup reads
+// consecutive sites, and sums the words of a run of equal labels in
+// registers (a tiled sheet's cells are taller than a run). Only where its
+// label changes does a run's total enter the workgroup's LDS hash table,
+// with LDS atomics, and each (label, word) of the table then costs one
+// global atomic. A label that finds no free slot goes straight to global
+// atomics. Integer sums, so the order does not matter. `gate` >= 0 makes
+// the launch a no-op unless flags[gate] is set, so a whole step can be
+// enqueued without host checks.
 constexpr int kSlots = 256;
 
-__device__ __forceinline__ void recount_body(const CellArgs &a, size_t n_sites, int sites_per_block,
-                                                      int moments_only, int gate, const int lb, const int gb) {
+__device__ __forceinline__ void recount_body(const CellArgs &a, size_t n_sites, int moments_only, int gate,
+                                             const int lb, const int gb) {
     if (gate >= 0 && a.flags[gate] == 0) return;
     __shared__ uint32_t keys[kSlots];
     __shared__ unsigned long long vals[kSlots * 12];
@@ -95,46 +101,7 @@ __device__ __forceinline__ void recount_body(const CellArgs &a, size_t n_sites, 
     for (int s = threadIdx.x; s < kSlots; s += blockDim.x) keys[s] = 0xFFFFFFFFu;
     for (int s = threadIdx.x; s < kSlots * cw; s += blockDim.x) vals[s] = 0;
     __syncthreads();
-    size_t lo = (size_t)lb * sites_per_block;
-    size_t hi = lo + sites_per_block;
-    if (hi > n_sites) hi = n_sites;
-    for (size_t i = lo + threadIdx.x; i < hi; i += blockDim.x) {
-        uint32_t word = a.labels[i];
-        uint32_t l = label_id(word);
-        int x, y, z;
-        coords(i, a.w, a.h, &x, &y, &z);
-        int64_t c[12];
-#pragma unroll
-        for (int q = 0; q < 12; ++q) c[q] = 0;
-        if (!moments_only) {
-            // neighbour offsets are within one site on every axis
-            int bonds = 0;
-            for (int k = 0; k < a.n_off; ++k) {
-                const int nx = wrap1(x + a.off[k][0], a.w), ny = wrap1(y + a.off[k][1], a.h),
-                          nz = wrap1(z + a.off[k][2], a.d);
-                const size_t nb = ((size_t)nz * a.h + ny) * (size_t)a.w + nx;
-                if (nb != i && a.labels[nb] != word) ++bonds;
-            }
-            c[0] = pack_vs(1, bonds);
-        }
-        if (cw > 1 && l != 0) {
-            int64_t ux = fold_int(x - a.anchor[3 * l], a.w);
-            int64_t uy = fold_int(y - a.anchor[3 * l + 1], a.h);
-            int64_t uz = fold_int(z - a.anchor[3 * l + 2], a.d);
-            c[1] = ux;
-            c[2] = uy;
-            c[3] = uz;
-            c[4] = ux * ux;
-            c[5] = uy * uy;
-            c[6] = uz * uz;
-            c[7] = ux * uy;
-            c[8] = ux * uz;
-            c[9] = uy * uz;
-            if (cw > 10) {
-                c[10] = a.nematic[2 * i];
-                c[11] = a.nematic[2 * i + 1];
-            }
-        }
+    auto flush = [&](uint32_t l, const int64_t *c) {
         uint32_t slot = hash32(l) & (kSlots - 1);
         int found = -1;
         for (int probe = 0; probe < kSlots; ++probe) {
@@ -154,6 +121,99 @@ __device__ __forceinline__ void recount_body(const CellArgs &a, size_t n_sites, 
                 atomicAdd(reinterpret_cast<unsigned long long *>(a.state + (size_t)l * cw + q),
                           (unsigned long long)c[q]);
         }
+    };
+    const int col_blocks = (a.w + kRecountCols - 1) / kRecountCols;
+    const int x0 = (lb % col_blocks) * kRecountCols;
+    const int x = x0 + (int)threadIdx.x;
+    const uint32_t rows = (uint32_t)a.h * (uint32_t)a.d;
+    const uint32_t r0 = (uint32_t)(lb / col_blocks) * kRecountRun;
+    // In a plane the bonds come from an LDS copy of the workgroup's rows
+    // with one row and one column of halo each side (periodic), loaded with
+    // consecutive sites per instruction, instead of eight global reads per
+    // site. Slot (j, c) holds row r0 - 1 + j, column x0 - 1 + c.
+    __shared__ uint32_t rtile[(kRecountRun + 2) * (kRecountCols + 2)];
+    constexpr int kRP = kRecountCols + 2;
+    const bool plane_bonds = !moments_only && a.d == 1;
+    const int ncols = a.w - x0 < kRecountCols ? a.w - x0 : kRecountCols;
+    if (plane_bonds) {
+        // lanes 0 .. ncols + 1 each load one tile column: the left halo, the
+        // workgroup's columns, the right halo
+        const int c = (int)threadIdx.x;
+        // the first row held (r0 - 1, wrapped), then a step at a time (a
+        // plane can be shorter than the rows held, so the wrap may repeat)
+        const int gy0 = (int)((r0 + (uint32_t)a.h - 1u) % (uint32_t)a.h);
+        for (int cc = c; cc < ncols + 2; cc += kRecountCols) {
+            const int gx = wrap1(x0 - 1 + cc, a.w);
+            uint32_t v[kRecountRun + 2];
+            int gy = gy0;
+#pragma unroll
+            for (int j = 0; j < kRecountRun + 2; ++j) {
+                v[j] = a.labels[(size_t)gy * (size_t)a.w + (size_t)gx];
+                gy = gy + 1 == a.h ? 0 : gy + 1;
+            }
+#pragma unroll
+            for (int j = 0; j < kRecountRun + 2; ++j) rtile[j * kRP + cc] = v[j];
+        }
+        __syncthreads();
+    }
+    if (x < a.w) {
+        int64_t acc[12];
+#pragma unroll
+        for (int q = 0; q < 12; ++q) acc[q] = 0;
+        uint32_t run_label = 0xFFFFFFFFu;
+        // (y, z) of the first row, then stepped
+        int y = (int)(r0 % (uint32_t)a.h), z = (int)(r0 / (uint32_t)a.h);
+        for (uint32_t r = r0; r < r0 + kRecountRun && r < rows; ++r) {
+            const size_t i = (size_t)r * (size_t)a.w + (size_t)x;
+            uint32_t word = plane_bonds ? rtile[(int)(r - r0 + 1) * kRP + (int)threadIdx.x + 1] : a.labels[i];
+            uint32_t l = label_id(word);
+            if (l != run_label) {
+                if (run_label != 0xFFFFFFFFu) flush(run_label, acc);
+#pragma unroll
+                for (int q = 0; q < 12; ++q) acc[q] = 0;
+                run_label = l;
+            }
+            if (plane_bonds) {
+                const int j = (int)(r - r0) + 1, c = (int)threadIdx.x + 1;
+                int bonds = 0;
+                for (int k = 0; k < a.n_off; ++k)
+                    if (rtile[(j + a.off[k][1]) * kRP + c + a.off[k][0]] != word) ++bonds;
+                acc[0] += pack_vs(1, bonds);
+            } else if (!moments_only) {
+                // neighbour offsets are within one site on every axis
+                int bonds = 0;
+                for (int k = 0; k < a.n_off; ++k) {
+                    const int nx = wrap1(x + a.off[k][0], a.w), ny = wrap1(y + a.off[k][1], a.h),
+                              nz = wrap1(z + a.off[k][2], a.d);
+                    const size_t nb = ((size_t)nz * a.h + ny) * (size_t)a.w + nx;
+                    if (nb != i && a.labels[nb] != word) ++bonds;
+                }
+                acc[0] += pack_vs(1, bonds);
+            }
+            if (cw > 1 && l != 0) {
+                int64_t ux = fold_int(x - a.anchor[3 * l], a.w);
+                int64_t uy = fold_int(y - a.anchor[3 * l + 1], a.h);
+                int64_t uz = fold_int(z - a.anchor[3 * l + 2], a.d);
+                acc[1] += ux;
+                acc[2] += uy;
+                acc[3] += uz;
+                acc[4] += ux * ux;
+                acc[5] += uy * uy;
+                acc[6] += uz * uz;
+                acc[7] += ux * uy;
+                acc[8] += ux * uz;
+                acc[9] += uy * uz;
+                if (cw > 10) {
+                    acc[10] += a.nematic[2 * i];
+                    acc[11] += a.nematic[2 * i + 1];
+                }
+            }
+            if (++y == a.h) {
+                y = 0;
+                ++z;
+            }
+        }
+        if (run_label != 0xFFFFFFFFu) flush(run_label, acc);
     }
     __syncthreads();
     for (int s = threadIdx.x; s < kSlots * cw; s += blockDim.x) {

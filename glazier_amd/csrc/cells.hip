@@ -28,9 +28,9 @@ __global__ void anchor_from_first_kernel(CellArgs a) {
     anchor_from_first_body(a, blockIdx.x, gridDim.x);
 }
 
-__global__ __launch_bounds__(256) void recount_kernel(CellArgs a, size_t n_sites, int sites_per_block,
-                                                      int moments_only, int gate) {
-    recount_body(a, n_sites, sites_per_block, moments_only, gate, blockIdx.x, gridDim.x);
+__global__ __launch_bounds__(kRecountCols) void recount_kernel(CellArgs a, size_t n_sites, int moments_only,
+                                                               int gate) {
+    recount_body(a, n_sites, moments_only, gate, blockIdx.x, gridDim.x);
 }
 
 __global__ void zero_moments_kernel(CellArgs a) {
@@ -127,8 +127,7 @@ hipError_t launch_consolidate(int64_t *a, int64_t *e, const uint32_t *n_labels, 
 }
 
 static void recount_gated(const CellArgs &a, size_t n_sites, int moments_only, int gate, hipStream_t s) {
-    const int per = 1024;
-    hipLaunchKernelGGL(recount_kernel, dim3(blocks_for(n_sites, per)), dim3(256), 0, s, a, n_sites, per,
+    hipLaunchKernelGGL(recount_kernel, dim3(recount_blocks(a.w, a.h, a.d)), dim3(kRecountCols), 0, s, a, n_sites,
                        moments_only, gate);
 }
 

@@ -163,6 +163,15 @@ struct FieldPlanes {
 };
 
 // Per-label bookkeeping launches (recounts, anchors, division/death).
+// A recount workgroup covers kRecountRun rows of kRecountCols columns (one
+// lane per column), summing runs of one label down a column in registers.
+constexpr int kRecountRun = 8;
+constexpr int kRecountCols = 256;
+inline uint32_t recount_blocks(int w, int h, int d) {
+    return (uint32_t)((w + kRecountCols - 1) / kRecountCols) *
+           (uint32_t)(((size_t)h * (size_t)d + kRecountRun - 1) / kRecountRun);
+}
+
 struct CellArgs {
     uint32_t *labels;      // packed words
     int64_t *state;        // consolidated per-label state
