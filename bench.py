@@ -113,6 +113,19 @@ def ensemble_bench(mcs: int):
     torch.cuda.synchronize()
     batched = time.perf_counter() - t0
     same = all((s.labels() == want).all() for s, want in zip(sims, looped_labels))
+    # algorithmic bytes of the batched run (the per-unit figures the single
+    # lattice's rooflines use): every colour pass reads the labels and the
+    # chemotactic fields, an accepted copy writes a label, a field step reads
+    # and writes each species and reads the owners, the moments read labels
+    accepted = 0
+    algo = 0.0
+    for (m, l, t), sim in zip(members, sims):
+        acc = sim.counters()[1]
+        accepted += acc
+        colours = 8 if m.depth > 1 else 4
+        chem = sum(1 for k in range(len(m.species)) if any(row[k] != 0.0 for row in m.chemotaxis))
+        per_mcs = colours * (4.0 + 4.0 * chem) + 8.0 * len(m.species) + (4.0 if m.species else 0.0) + 4.0
+        algo += per_mcs * l.size * mcs + 4.0 * acc
     # the captured step alone, after tuning
     t0 = time.perf_counter()
     batch.step(10)
@@ -130,10 +143,42 @@ def ensemble_bench(mcs: int):
         "batched_replicas_per_s": n / batched, "looped_replicas_per_s": n / looped,
         "speedup": looped / batched, "batched_s": batched, "looped_s": looped,
         "batched_site_attempts_per_s": sites * mcs / batched,
+        "batched_hbm_frac_algorithmic": algo / batched / (HBM_PEAK_GBS * 1e9),
+        "batched_algorithmic_bytes": algo,
         "graph_step_ms": graph_step * 1e3, "lattices_identical": bool(same),
-        "tiles": [{"dims": g["dims"], "lattices": g["lattices"], "tile": g["tile"],
-                   "threads": g["threads"]} for g in info["groups"]],
+        "tiles": [{k: g[k] for k in ("dims", "lattices", "tile", "threads", "wgs_per_cu") if k in g}
+                  for g in info["groups"]],
     }
+
+
+def cpu_baseline_ensemble(steps: int = 10):
+    """The serial oracle on the same 80-member ensemble, one member per host
+    core at a time across every core: each core takes the next member and
+    runs `steps` of its steps. Replicas/s extrapolates the wall time to the
+    full 100 steps a replica runs."""
+    import os
+    from concurrent.futures import ThreadPoolExecutor
+
+    import oracle
+    from glazier_amd.workloads import ensemble
+
+    members = ensemble(seed=3)
+    nproc = os.cpu_count() or 1
+    # the cores this process may run on, at most 16 (one GPU's share of a
+    # shared host)
+    cores = max(1, min(16, len(os.sched_getaffinity(0))))
+    sims = [oracle.SerialSimulation(m, l, t) for m, l, t in members]
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(max_workers=cores) as pool:
+        list(pool.map(lambda sim: sim.step(steps), sims))  # ctypes drops the GIL in the call
+    wall = time.perf_counter() - t0
+    for sim in sims:
+        sim.close()
+    return {"value": len(members) / (wall * 100.0 / steps), "unit": "replicas/s", "cores": cores,
+            "nproc": nproc, "kind": "port",
+            "sample": f"serial f64 oracle, all {len(members)} ensemble members for {steps} of their 100 "
+                      f"steps on {cores} host threads (nproc {nproc}), one member per thread at a time, "
+                      f"{wall:.2f} s, scaled to 100 steps"}
 
 
 def adjoint_bench(cpu: bool, adam_steps: int = 50):
@@ -460,6 +505,41 @@ def measured_traffic(side: int):
             "fetch_scale": f_scale, "write_scale": w_scale}
 
 
+def moving_roofline(side: int):
+    """The same pass on the reference's benchmark model (examples/bench.rs:
+    7-25: contact 16/8, no surface term), whose sheet moves (about 7% of
+    attempts accepted): one event pair around 25 steps (100 back-to-back
+    passes) on the engine's stream, bytes per launch 4 N + 4 per accepted
+    copy."""
+    import torch
+
+    from glazier_amd import HipSimulation
+    from glazier_amd.workloads import reference_model
+
+    sim = HipSimulation(*reference_model(side, 1))
+    sim.step(8)
+    torch.cuda.synchronize()
+    acc0 = sim.counters()[1]
+    sim.set_timing(2)
+    sim.step(25)
+    sim.set_timing(False)
+    ms, launches = sim.kernel_time("step")
+    acc = sim.counters()[1] - acc0
+    sim.close()
+    n = side * side
+    avg_s = ms / launches / 1e3
+    bytes_per_launch = 4.0 * n + 4.0 * acc / launches
+    achieved = bytes_per_launch / avg_s / 1e9
+    return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "kernel": "sweep_plain2d_kernel (one colour pass)",
+            "workload": f"reference benchmark model {side}^2, 8x8 cells, contact 16/8, V_t=64, "
+                        "lambda_V=2, T=10, Moore, seed 1",
+            "avg_launch_us": avg_s * 1e6, "algorithmic_bytes_per_launch": bytes_per_launch,
+            "acceptance": acc / (25.0 * n),
+            "avg_launch_us_method": "HIP events around 100 back-to-back launches on the engine's "
+                                    "stream, divided by 100"}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -602,6 +682,7 @@ def main() -> int:
     }
     for s in sims:
         s.close()
+    line["roofline_moving"] = moving_roofline(side)
     if not args.no_traffic:
         traffic = measured_traffic(side)
         if traffic is not None:
@@ -621,6 +702,8 @@ def main() -> int:
         line["cpu_baseline"] = cpu_baseline(side, args.cpu_seconds)
         if leg is not None:
             line["cpu_baseline"]["adjoint"] = cpu_baseline_adjoint(leg)
+        if not args.no_ensemble:
+            line["cpu_baseline"]["ensemble"] = cpu_baseline_ensemble()
         if not args.no_configs:
             line["cpu_baseline"]["configs"] = cpu_baseline_configs()
             line["cpu_baseline"]["engine_comparison"] = cpu_baseline_comparison()

@@ -38,6 +38,30 @@ const Tile kTiles2D[kCandidates] = {
 const Tile kTiles3D[kCandidates] = {
     {16, 4, 4, 256}, {8, 8, 4, 256}, {16, 4, 8, 512}, {8, 4, 4, 128}, {16, 8, 4, 512}};
 
+// Interior tiles of the field launches in sites, timed on the same steps as
+// the colour passes' candidates (a field step's bits do not depend on its
+// tile). The first is the single lattice's own choice.
+const Tile kFieldTiles2D[kCandidates] = {
+    {64, 32, 1, 256}, {128, 16, 1, 256}, {32, 32, 1, 256}, {64, 64, 1, 256}, {128, 32, 1, 256}};
+const Tile kFieldTiles3D[kCandidates] = {
+    {16, 8, 8, 256}, {32, 8, 4, 256}, {16, 16, 4, 256}, {8, 8, 8, 256}, {32, 4, 8, 256}};
+
+// Occupancy candidates, timed on the chosen tiles: at most this many
+// workgroups of one launch per CU (0: as many as registers and LDS allow).
+// A cap is reached by asking for more LDS than the kernel uses, so that one
+// more workgroup would not fit in the CU's 160 KiB; every cap stays within
+// the 64 KiB a workgroup may take.
+constexpr int kOccupancy = 4;
+const int kWgsPerCu[kOccupancy] = {0, 4, 3, 2};
+constexpr size_t kLdsPerCu = 160 * 1024;
+
+size_t occupancy_lds(size_t lds, int occ) {
+    const int wgs = kWgsPerCu[occ];
+    if (wgs == 0) return lds;
+    const size_t need = (kLdsPerCu / (size_t)(wgs + 1) + 16) & ~(size_t)15;
+    return std::max(lds, need);
+}
+
 // The candidate clamped to the lattice; spare lanes go to the next axis.
 void clamp_tile(const ghip_sim *s, const Tile &t, int *tx, int *ty, int *tz) {
     *tx = std::min(t.tx, pow2_floor(s->W / 2));
@@ -87,6 +111,8 @@ struct SweepGroup {
     std::vector<uint32_t> reps;  // largest first
     int cand = 0;
     float ms[kCandidates] = {};
+    int occ = 0;
+    float occ_ms[kOccupancy] = {};
     int threads = 0;
     size_t lds = 0;
     ListRef ref;
@@ -96,7 +122,7 @@ struct SweepGroup {
 struct FieldLaunch {
     bool d3 = false;
     int chunk = 0;
-    size_t lds = 0;
+    size_t lds = 0;  // what the kernel uses; the launch adds the occupancy cap
     ListRef ref;
     WorkList w{};
 };
@@ -131,8 +157,13 @@ struct ghip_batch {
     int max_colours = 0, max_chunks = 0;
     bool any_cells = false, any_mot = false;
     std::vector<uint64_t> expect_mcs;  // each lattice's step count as the batch left it
-    int tune_step = 0;                 // 0: warm-up, 1..kCandidates: candidate - 1
+    // 0: warm-up; 1..kCandidates: tile candidate tune_step - 1 (colour passes
+    // and fields); then occupancy candidate tune_step - kCandidates
+    int tune_step = 0;
     bool tuned = false;
+    // the field launches' tile and occupancy candidates, per dimension
+    int field_cand[2] = {0, 0}, field_occ[2] = {0, 0};
+    float field_ms[2][kCandidates] = {}, field_occ_ms[2][kOccupancy] = {};
     bool autotune = true;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -204,6 +235,19 @@ SweepArgs sweep_for(const ghip_sim *s, int cand, uint32_t *blocks) {
     return a;
 }
 
+// The field tile of lattice s under field candidate `cand`, clamped to the
+// lattice, into its replica record; returns the workgroups per species.
+uint32_t field_tile(const ghip_sim *s, int cand, BatchReplica *B) {
+    const Tile &t = (s->d3 ? kFieldTiles3D : kFieldTiles2D)[cand];
+    B->fx_log2 = ilog2(std::min(t.tx, pow2_floor(s->W)));
+    B->fy_log2 = ilog2(std::min(t.ty, pow2_floor(s->H)));
+    B->fz = s->d3 ? std::min(t.tz, pow2_floor(s->D)) : 1;
+    B->f_tiles_x = (s->W + (1 << B->fx_log2) - 1) >> B->fx_log2;
+    B->f_tiles_y = (s->H + (1 << B->fy_log2) - 1) >> B->fy_log2;
+    const uint32_t tiles_z = s->d3 ? cdiv(s->D, B->fz) : 1;
+    return (uint32_t)B->f_tiles_x * B->f_tiles_y * tiles_z;
+}
+
 // Rebuild every replica record and work list from the lattices' current
 // state and the groups' current tile candidates, and upload them.
 ghip_status rebuild(ghip_batch *b) {
@@ -211,7 +255,7 @@ ghip_status rebuild(ghip_batch *b) {
     b->drop_graph();
     const size_t R = b->sims.size();
     b->pool.clear();
-    std::vector<uint32_t> sweep_blocks(R, 0);
+    std::vector<uint32_t> sweep_blocks(R, 0), field_blocks(R, 0);
     for (auto &g : b->groups) {
         g.lds = 0;
         g.threads = (g.d3 ? kTiles3D : kTiles2D)[g.cand].threads;
@@ -266,11 +310,7 @@ ghip_status rebuild(ghip_batch *b) {
             B.fd[k] = s->fd[k];
             B.fk[k] = s->fk[k];
         }
-        B.fx_log2 = s->fx_log2;
-        B.fy_log2 = s->fy_log2;
-        B.fz = s->fz;
-        B.f_tiles_x = s->f_tiles_x;
-        B.f_tiles_y = s->f_tiles_y;
+        field_blocks[r] = field_tile(s, b->field_cand[s->d3 ? 1 : 0], &B);
     }
     // field launches: chunk c of every species of every lattice of one dimension
     b->fields.clear();
@@ -287,11 +327,11 @@ ghip_status rebuild(ghip_batch *b) {
                 for (uint32_t k = 0; k < s->m.n_species; ++k) {
                     if (b->reps[r].chunks[k] <= c) continue;
                     members.push_back((uint32_t)r * S + k);
-                    blocks.push_back((uint32_t)s->f_blocks);
+                    blocks.push_back(field_blocks[r]);
                     FieldArgs f{};
-                    f.ix_log2 = s->fx_log2;
-                    f.iy_log2 = s->fy_log2;
-                    f.iz = s->fz;
+                    f.ix_log2 = b->reps[r].fx_log2;
+                    f.iy_log2 = b->reps[r].fy_log2;
+                    f.iz = b->reps[r].fz;
                     f.steps = b->reps[r].chunk_steps[k][c];
                     fl.lds = std::max(fl.lds, field_lds_bytes(f, s->d3));
                 }
@@ -362,9 +402,16 @@ ghip_status rebuild(ghip_batch *b) {
     return GHIP_OK;
 }
 
-// One step of every lattice. With `timed`, events bracket each sweep group's
-// launches and are appended to `marks` as (group, start, stop).
-ghip_status enqueue_step(ghip_batch *b, std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> *marks) {
+// A timed launch: a sweep group (kind 0, its index) or a field launch (kind
+// 1, its dimension's index: 0 planes, 1 volumes).
+struct Mark {
+    int kind, idx;
+    hipEvent_t start, stop;
+};
+
+// One step of every lattice. With `marks`, events bracket each sweep and
+// field launch and are appended to it.
+ghip_status enqueue_step(ghip_batch *b, std::vector<Mark> *marks) {
     const BatchArgs &A = b->args;
     hipStream_t s = b->stream;
     size_t ev = 0;
@@ -390,17 +437,31 @@ ghip_status enqueue_step(ghip_batch *b, std::vector<std::pair<int, std::pair<hip
                 HIP_TRY(next_event(&e1));
                 HIP_TRY(hipEventRecord(e0, s));
             }
-            HIP_TRY(launch_batch_sweep(A, G.d3, G.full, G.moore, G.w, G.threads, G.lds, s));
+            HIP_TRY(launch_batch_sweep(A, G.d3, G.full, G.moore, G.w, G.threads, occupancy_lds(G.lds, G.occ),
+                                       s));
             if (marks) {
                 HIP_TRY(hipEventRecord(e1, s));
-                marks->push_back({(int)g, {e0, e1}});
+                marks->push_back({0, (int)g, e0, e1});
             }
         }
     }
     for (int c = 0; c < b->max_chunks; ++c) {
         HIP_TRY(launch_batch_prepare_field(A, c, s));
-        for (const auto &f : b->fields)
-            if (f.chunk == c) HIP_TRY(launch_batch_field(A, f.d3, f.w, f.lds, s));
+        for (const auto &f : b->fields) {
+            if (f.chunk != c) continue;
+            const int d = f.d3 ? 1 : 0;
+            hipEvent_t e0 = nullptr, e1 = nullptr;
+            if (marks) {
+                HIP_TRY(next_event(&e0));
+                HIP_TRY(next_event(&e1));
+                HIP_TRY(hipEventRecord(e0, s));
+            }
+            HIP_TRY(launch_batch_field(A, f.d3, f.w, occupancy_lds(f.lds, b->field_occ[d]), s));
+            if (marks) {
+                HIP_TRY(hipEventRecord(e1, s));
+                marks->push_back({1, d, e0, e1});
+            }
+        }
     }
     if (b->any_cells || b->any_mot) {
         HIP_TRY(launch_batch_prepare_cells(A, s));
@@ -544,8 +605,27 @@ ghip_status ghip_batch_set_tiles(ghip_batch *b, int candidate) {
     }
     b->autotune = false;
     b->tuned = true;
-    for (auto &g : b->groups) g.cand = candidate;
+    for (auto &g : b->groups) {
+        g.cand = candidate;
+        g.occ = 0;
+    }
+    for (int d = 0; d < 2; ++d) {
+        b->field_cand[d] = candidate;
+        b->field_occ[d] = 0;
+    }
     return rebuild(b);
+}
+
+ghip_status ghip_batch_set_occupancy(ghip_batch *b, int candidate) {
+    if (!b) return fail(GHIP_BAD_ARG, "null handle");
+    if (candidate < 0 || candidate >= kOccupancy) return fail(GHIP_BAD_ARG, "no such occupancy candidate");
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    b->drop_graph();
+    b->autotune = false;
+    b->tuned = true;
+    for (auto &g : b->groups) g.occ = candidate;
+    for (int d = 0; d < 2; ++d) b->field_occ[d] = candidate;
+    return GHIP_OK;
 }
 
 }  // extern "C"
@@ -555,44 +635,75 @@ namespace {
 // `steps` steps of every lattice, none of which can run out of label ids.
 ghip_status batch_run(ghip_batch *b, uint64_t steps) {
     uint64_t q = 0;
-    // tuning: real steps, one candidate shape each, timed per group
+    // tuning: real steps, one candidate each, timed per sweep group and per
+    // dimension's field launches; first the tiles (from the warm-up on, with
+    // no occupancy cap), then the occupancy caps on the fastest tiles
     while (q < steps && b->autotune && !b->tuned) {
-        const int cand = b->tune_step == 0 ? 0 : b->tune_step - 1;
+        const int t = b->tune_step;
+        const bool tiles = t <= kCandidates;
+        const int cand = t == 0 ? 0 : t - 1;
+        const int occ = tiles ? 0 : t - kCandidates;
         bool reshape = false;
-        for (auto &g : b->groups)
-            if (g.cand != cand) {
+        for (auto &g : b->groups) {
+            if (tiles && g.cand != cand) {
                 g.cand = cand;
                 reshape = true;
             }
+            g.occ = occ;
+        }
+        for (int d = 0; d < 2; ++d) {
+            if (tiles && b->field_cand[d] != cand) {
+                b->field_cand[d] = cand;
+                reshape = true;
+            }
+            b->field_occ[d] = occ;
+        }
         if (reshape) {
             ghip_status st = rebuild(b);
             if (st != GHIP_OK) return st;
         }
-        std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> marks;
+        std::vector<Mark> marks;
         ghip_status st = enqueue_step(b, &marks);
         if (st != GHIP_OK) return st;
         HIP_TRY(hipStreamSynchronize(b->stream));
-        if (b->tune_step > 0) {
-            for (auto &g : b->groups) g.ms[cand] = 0.0f;
-            for (auto &mk : marks) {
+        if (t > 0) {
+            for (auto &g : b->groups) (tiles ? g.ms[cand] : g.occ_ms[occ]) = 0.0f;
+            for (int d = 0; d < 2; ++d) (tiles ? b->field_ms[d][cand] : b->field_occ_ms[d][occ]) = 0.0f;
+            for (const Mark &mk : marks) {
                 float ms = 0.0f;
-                HIP_TRY(hipEventElapsedTime(&ms, mk.second.first, mk.second.second));
-                b->groups[mk.first].ms[cand] += ms;
+                HIP_TRY(hipEventElapsedTime(&ms, mk.start, mk.stop));
+                if (mk.kind == 0)
+                    (tiles ? b->groups[mk.idx].ms[cand] : b->groups[mk.idx].occ_ms[occ]) += ms;
+                else
+                    (tiles ? b->field_ms[mk.idx][cand] : b->field_occ_ms[mk.idx][occ]) += ms;
             }
         }
         advance_host(b, 1);
         b->eager_steps += 1;
         ++q;
-        if (++b->tune_step > kCandidates) {
+        ++b->tune_step;
+        auto fastest = [](const float *ms, int n) {
+            int best = 0;
+            for (int c = 1; c < n; ++c)
+                if (ms[c] < ms[best]) best = c;
+            return best;
+        };
+        if (b->tune_step == kCandidates + 1) {
+            // the tiles are chosen; their uncapped time is occupancy candidate 0
             for (auto &g : b->groups) {
-                int best = 0;
-                for (int c = 1; c < kCandidates; ++c)
-                    if (g.ms[c] < g.ms[best]) best = c;
-                g.cand = best;
+                g.cand = fastest(g.ms, kCandidates);
+                g.occ_ms[0] = g.ms[g.cand];
             }
-            b->tuned = true;
+            for (int d = 0; d < 2; ++d) {
+                b->field_cand[d] = fastest(b->field_ms[d], kCandidates);
+                b->field_occ_ms[d][0] = b->field_ms[d][b->field_cand[d]];
+            }
             st = rebuild(b);
             if (st != GHIP_OK) return st;
+        } else if (b->tune_step == kCandidates + kOccupancy) {
+            for (auto &g : b->groups) g.occ = fastest(g.occ_ms, kOccupancy);
+            for (int d = 0; d < 2; ++d) b->field_occ[d] = fastest(b->field_occ_ms[d], kOccupancy);
+            b->tuned = true;
         }
     }
     if (q < steps) {
@@ -661,6 +772,15 @@ ghip_status ghip_batch_step(ghip_batch *b, uint64_t steps) {
 
 ghip_status ghip_batch_describe(const ghip_batch *b, char *out, size_t n) {
     if (!b || !out || n == 0) return fail(GHIP_BAD_ARG, "null argument");
+    auto list = [](const float *ms, int k) {
+        std::string s;
+        char buf[32];
+        for (int c = 0; c < k; ++c) {
+            std::snprintf(buf, sizeof buf, "%s%.4f", c ? ", " : "", ms[c]);
+            s += buf;
+        }
+        return s;
+    };
     std::string s = "{\"lattices\": " + std::to_string(b->sims.size()) +
                     ", \"tuned\": " + (b->tuned ? "true" : "false") +
                     ", \"graph_steps\": " + std::to_string(b->graph_steps) +
@@ -668,19 +788,33 @@ ghip_status ghip_batch_describe(const ghip_batch *b, char *out, size_t n) {
     for (size_t g = 0; g < b->groups.size(); ++g) {
         const SweepGroup &G = b->groups[g];
         const Tile &t = (G.d3 ? kTiles3D : kTiles2D)[G.cand];
-        char buf[256];
+        char buf[320];
         std::snprintf(buf, sizeof buf,
                       "%s{\"dims\": %d, \"full\": %s, \"lattices\": %zu, \"tile\": [%d, %d, %d], "
-                      "\"threads\": %d, \"blocks_per_pass\": %u, \"tuning_ms\": [",
+                      "\"threads\": %d, \"blocks_per_pass\": %u, \"wgs_per_cu\": %d, \"lds_bytes\": %zu, "
+                      "\"tuning_ms\": [",
                       g ? ", " : "", G.d3 ? 3 : 2, G.full ? "true" : "false", G.reps.size(), t.tx, t.ty,
-                      t.tz, t.threads, G.ref.blocks);
+                      t.tz, t.threads, G.ref.blocks, kWgsPerCu[G.occ], occupancy_lds(G.lds, G.occ));
         s += buf;
-        for (int c = 0; c < kCandidates; ++c) {
-            std::snprintf(buf, sizeof buf, "%s%.4f", c ? ", " : "", G.ms[c]);
-            s += buf;
-        }
-        s += "]}";
+        s += list(G.ms, kCandidates) + "], \"occupancy_ms\": [" + list(G.occ_ms, kOccupancy) + "]}";
     }
+    s += "], \"fields\": [";
+    bool first = true;
+    for (int d = 0; d < 2; ++d) {
+        bool any = false;
+        for (const auto &f : b->fields) any = any || (f.d3 ? 1 : 0) == d;
+        if (!any) continue;
+        const Tile &t = (d ? kFieldTiles3D : kFieldTiles2D)[b->field_cand[d]];
+        char buf[192];
+        std::snprintf(buf, sizeof buf, "%s{\"dims\": %d, \"tile\": [%d, %d, %d], \"wgs_per_cu\": %d, \"tuning_ms\": [",
+                      first ? "" : ", ", d ? 3 : 2, t.tx, t.ty, t.tz, kWgsPerCu[b->field_occ[d]]);
+        s += buf;
+        s += list(b->field_ms[d], kCandidates) + "], \"occupancy_ms\": [" +
+             list(b->field_occ_ms[d], kOccupancy) + "]}";
+        first = false;
+    }
+    s += "], \"wgs_per_cu_candidates\": [";
+    for (int c = 0; c < kOccupancy; ++c) s += (c ? ", " : "") + std::to_string(kWgsPerCu[c]);
     s += "]}";
     if (s.size() + 1 > n) return fail(GHIP_BAD_ARG, "the buffer is too small");
     std::memcpy(out, s.c_str(), s.size() + 1);

@@ -87,6 +87,7 @@ def library() -> ctypes.CDLL:
     lib.ghip_batch_destroy.argtypes = [ctypes.c_void_p]
     lib.ghip_batch_step.argtypes = [ctypes.c_void_p, ctypes.c_uint64]
     lib.ghip_batch_set_tiles.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    lib.ghip_batch_set_occupancy.argtypes = [ctypes.c_void_p, ctypes.c_int]
     lib.ghip_batch_describe.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_size_t]
     lib.ghip_last_error.restype = ctypes.c_char_p
     _lib = lib
@@ -287,6 +288,11 @@ class HipBatch:
     def set_tiles(self, candidate: int) -> None:
         """Fix every group to tile candidate 0..4, or -1 to time them again."""
         _check(library().ghip_batch_set_tiles(self._h, int(candidate)))
+
+    def set_occupancy(self, candidate: int) -> None:
+        """Fix every launch to occupancy candidate 0..3 (no cap, or at most
+        4, 3 or 2 workgroups per CU) on the current tiles."""
+        _check(library().ghip_batch_set_occupancy(self._h, int(candidate)))
 
     def describe(self) -> dict:
         import json

@@ -213,13 +213,20 @@ ghip_status ghip_batch_create(ghip_sim *const *sims, size_t n, ghip_batch **out)
 ghip_status ghip_batch_destroy(ghip_batch *batch);
 
 /* Runs `steps` Monte Carlo steps of every lattice in the batch. The first
- * six steps of a new batch time each candidate tile shape of the colour
- * passes on real steps and keep the fastest per kernel variant. */
+ * nine steps of a new batch are timed: after a warm-up, each of five tile
+ * shapes of the colour passes and of the field launches, then each of three
+ * occupancy caps (at most 4, 3 or 2 workgroups of a launch per CU) on the
+ * fastest tiles; the fastest tile and cap per kernel variant are kept. */
 ghip_status ghip_batch_step(ghip_batch *batch, uint64_t steps);
 
-/* Fixes every group to one tile candidate (0..4) and skips the timing
- * steps, or with -1 starts the timing over. */
+/* Fixes every group to one tile candidate (0..4) with no occupancy cap and
+ * skips the timing steps, or with -1 starts the timing over. */
 ghip_status ghip_batch_set_tiles(ghip_batch *batch, int candidate);
+
+/* Fixes every launch to one occupancy candidate (0: no cap; 1..3: at most
+ * 4, 3 or 2 workgroups per CU) on the current tiles and skips the timing
+ * steps. */
+ghip_status ghip_batch_set_occupancy(ghip_batch *batch, int candidate);
 
 /* A JSON description of the groups, their chosen tiles and timings. */
 ghip_status ghip_batch_describe(const ghip_batch *batch, char *out, size_t n);

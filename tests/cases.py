@@ -10,7 +10,7 @@ from __future__ import annotations
 import numpy as np
 
 from glazier_amd.model import CellType, Model, Species, default_cell_types, tiled_labels
-from glazier_amd.workloads import infection_sheet, plain_monolayer
+from glazier_amd.workloads import infection_model, infection_sheet, plain_monolayer
 
 
 def ref_model(w=64, h=64, d=1, seed=1, order=2, contact=(0.0, 16.0, 16.0, 8.0), temperature=10.0,
@@ -161,6 +161,14 @@ def case(name):
     if name == "infection128":
         m, l, t = infection_sheet(128, 7)
         return m, l, t, None, None, 30
+    if name == "infection3d":
+        # the ensemble's volume physics on a 32^3 lattice of 8^3 cells, every
+        # eighth cell infected
+        m = infection_model(32, 32, 32, 64, target_volume=512.0)
+        labels = tiled_labels(m, 8, 4, 4, 4)
+        kinds = np.ones(64, dtype=np.uint8)
+        kinds[::8] = 2
+        return m, labels, default_cell_types(65, kinds), None, None, 4
     if name == "motility3d":
         m = ref_model(16, 16, 16, seed=53, max_activity=20.0, lambda_activity=60.0)
         m.types[1].target_volume = 512.0
@@ -205,5 +213,6 @@ PARITY_CASES = ["plain128", "surface256", "surface1024", "scattered1024", "fast_
                 "fast_lone_typed", "ref64", "medium_gaps", "rect64x32",
                 "von_neumann", "float_energies", "volume3d_o2", "volume3d_o3", "volume3d_o1",
                 "connected", "connected3d", "chemotaxis_ramp", "secreting", "length",
-                "motility", "drift", "death", "division", "compaction", "infection128", "motility3d",
+                "motility", "drift", "death", "division", "compaction", "infection128", "infection3d",
+                "motility3d",
                 "length3d", "division3d", "nematic", "nematic_division", "nematic3d"]

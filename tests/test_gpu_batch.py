@@ -18,7 +18,7 @@ pytestmark = pytest.mark.gpu
 BATCH_CASES = ["plain128", "fast_medium_typed", "medium_gaps", "rect64x32", "von_neumann",
                "float_energies", "volume3d_o2", "volume3d_o3", "volume3d_o1", "connected",
                "connected3d", "chemotaxis_ramp", "secreting", "length", "motility", "drift",
-               "death", "division", "infection128", "motility3d", "length3d", "division3d",
+               "death", "division", "infection128", "infection3d", "motility3d", "length3d", "division3d",
                "nematic", "nematic_division", "nematic3d"]
 
 
@@ -51,15 +51,19 @@ def test_a_batch_matches_each_lattice_alone(engine, oracle_lib):
     from glazier_amd import HipBatch
 
     runs = build(BATCH_CASES)
-    print(f"[batch] {len(runs)} lattices, 2 + 9 steps", flush=True)
+    print(f"[batch] {len(runs)} lattices, 2 + 12 steps", flush=True)
     batch = HipBatch([r[2] for r in runs])
-    batch.step(2)   # warm-up and the first timed candidate
-    batch.step(9)   # the other candidates, then the captured step
+    batch.step(2)   # warm-up and the first timed tile candidate
+    batch.step(12)  # the other tiles, the occupancy caps, then the captured step
     info = batch.describe()
-    assert info["tuned"] and info["graph_steps"] == 5 and info["eager_steps"] == 6
+    assert info["tuned"] and info["graph_steps"] == 5 and info["eager_steps"] == 9
+    assert {f["dims"] for f in info["fields"]} == {2, 3}
+    for g in info["groups"] + info["fields"]:
+        assert g["wgs_per_cu"] in info["wgs_per_cu_candidates"]
+        assert all(ms > 0 for ms in g["tuning_ms"] + g["occupancy_ms"])
     for name, m, gpu, emu in runs:
-        emu.step(11)
-        assert gpu.mcs == 11
+        emu.step(14)
+        assert gpu.mcs == 14
         assert_same(name, m, gpu, emu)
     batch.close()
 
@@ -68,7 +72,8 @@ def test_a_batch_matches_each_lattice_alone(engine, oracle_lib):
 def test_every_tile_candidate_gives_the_same_lattice(engine, oracle_lib, candidate):
     from glazier_amd import HipBatch
 
-    runs = build(["surface256", "length", "volume3d_o2", "division", "motility3d"])
+    runs = build(["surface256", "length", "volume3d_o2", "division", "motility3d", "infection128",
+                  "infection3d"])
     print(f"[batch] tile candidate {candidate}", flush=True)
     batch = HipBatch([r[2] for r in runs])
     batch.set_tiles(candidate)
@@ -76,6 +81,26 @@ def test_every_tile_candidate_gives_the_same_lattice(engine, oracle_lib, candida
     assert batch.describe()["graph_steps"] == 4
     for name, m, gpu, emu in runs:
         emu.step(4)
+        assert_same(name, m, gpu, emu)
+    batch.close()
+
+
+@pytest.mark.parametrize("candidate", range(1, 4))
+def test_every_occupancy_cap_gives_the_same_lattice(engine, oracle_lib, candidate):
+    from glazier_amd import HipBatch
+
+    runs = build(["infection128", "infection3d", "volume3d_o2", "chemotaxis_ramp", "division3d"])
+    print(f"[batch] occupancy candidate {candidate}", flush=True)
+    batch = HipBatch([r[2] for r in runs])
+    batch.set_tiles(1)
+    batch.set_occupancy(candidate)
+    batch.step(3)
+    info = batch.describe()
+    assert info["graph_steps"] == 3
+    want = info["wgs_per_cu_candidates"][candidate]
+    assert all(g["wgs_per_cu"] == want for g in info["groups"] + info["fields"])
+    for name, m, gpu, emu in runs:
+        emu.step(3)
         assert_same(name, m, gpu, emu)
     batch.close()
 
