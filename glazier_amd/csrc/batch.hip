@@ -80,7 +80,7 @@ __global__ void batch_prepare_sweep_kernel(BatchArgs b, int phase) {
 }
 
 template <bool D3, bool FULL, bool MOORE2D>
-__global__ __launch_bounds__(512) void batch_sweep_kernel(BatchArgs b, WorkList w) {
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(D3 ? 5 : 4))) void batch_sweep_kernel(BatchArgs b, WorkList w) {
     const uint32_t g = find_entry(w, blockIdx.x);
     const SweepArgs &a = b.cur_sweep[w.members[g]];
     if (a.n_off == 0) return;

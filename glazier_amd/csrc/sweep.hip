@@ -5,7 +5,7 @@
 namespace ghip {
 
 template <bool D3, bool FULL, bool MOORE2D>
-__global__ __launch_bounds__(512) void sweep_kernel(SweepArgs a) {
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(D3 ? 5 : 4))) void sweep_kernel(SweepArgs a) {
     sweep_body<D3, FULL, MOORE2D>(a, blockIdx.x);
 }
 

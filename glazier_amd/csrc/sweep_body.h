@@ -95,44 +95,20 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &a, const int blk) {
     uint32_t *sLab = reinterpret_cast<uint32_t *>(sOff + 64);
     float *sAct = reinterpret_cast<float *>(sLab + LX * LY * LZ);
 
-    for (int i = tid; i < T * T; i += nthreads) sJ[i] = tab->J[i / T][i % T];
-    for (int i = tid; i < T; i += nthreads) {
-        sP[4 * i + 0] = tab->lam_v[i];
-        sP[4 * i + 1] = tab->tgt_v[i];
-        sP[4 * i + 2] = tab->lam_s[i];
-        sP[4 * i + 3] = tab->tgt_s[i];
-    }
-    for (int i = tid; i < (int)a.n_thr; i += nthreads) sThr[i] = tab->thr[i];
-    if (tid < a.n_off) {
-        sOff[tid] = a.off[tid][0] + a.off[tid][1] * LX + a.off[tid][2] * LX * LY;
-        sOff[32 + tid] = (a.off[tid][0] + 1) | ((a.off[tid][1] + 1) << 2) | ((a.off[tid][2] + 1) << 4);
-    }
-
-    // ---- fold this block's share of the per-label snapshot for the next
-    // pass, and clear the delta buffer two passes ahead.
-    {
-        const uint64_t words = (uint64_t)(*a.n_labels) * (uint64_t)a.cell_words;
-        uint64_t lo = (uint64_t)blk * a.fold_per_block;
-        uint64_t hi = lo + a.fold_per_block;
-        if (hi > words) hi = words;
-        for (uint64_t i = lo + tid; i < hi; i += nthreads) {
-            a.a_next[i] = a.a_cur[i] + a.e_prev[i];
-            a.g_zero[i] = 0;
-        }
-    }
-
-    // ---- stage labels (and activity): the tile with its halo as one flat
-    // range over all lanes, four loads in flight per lane before the LDS
-    // stores wait on them (a row at a time leaves each wave waiting on one
-    // load per row).
+    // ---- every load of the prologue goes out before the first wait on one:
+    // the labels (and activity) of the tile with its halo, as one flat range
+    // over all lanes, up to kStage loads per lane; then this block's share of
+    // the snapshot fold (two words per lane now, stored at the end of the
+    // pass); then the tables bound for LDS. A wavefront's wait for a load is
+    // a wait for every load it issued before, so the chain is one round trip
+    // where loading the tables, folding and staging in turn made it three.
     const uint32_t *src_lab = (FULL && (a.flags & kMotility)) ? a.labels_prev : a.labels;
     const bool with_act = FULL && (a.flags & kMotility);
     const int staged = LX * LY * LZ;
-    for (int base = 0; base < staged; base += 4 * nthreads) {
-        uint32_t lv[4];
-        float av[4];
+    constexpr int kStage = D3 ? 4 : 8;
+    auto stage_load = [&](int base, uint32_t (&lv)[kStage], float (&av)[kStage]) {
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < kStage; ++u) {
             const int e = base + u * nthreads + tid;
             if (e < staged) {
                 const int r = e / LX, c = e - r * LX;
@@ -145,14 +121,57 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &a, const int blk) {
                 if (with_act) av[u] = a.act_prev[g];
             }
         }
+    };
+    auto stage_store = [&](int base, const uint32_t (&lv)[kStage], const float (&av)[kStage]) {
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < kStage; ++u) {
             const int e = base + u * nthreads + tid;
             if (e < staged) {
                 sLab[e] = lv[u];
                 if (with_act) sAct[e] = av[u];
             }
         }
+    };
+    uint32_t lv0[kStage];
+    float av0[kStage];
+    stage_load(0, lv0, av0);
+
+    // the fold of this block's share of the per-label snapshot for the next
+    // pass (a_next = a_cur + e_prev), and the clearing of the delta buffer
+    // two passes ahead: nothing in the pass reads either buffer it writes
+    const uint64_t words = (uint64_t)(*a.n_labels) * (uint64_t)a.cell_words;
+    const uint64_t f_lo = (uint64_t)blk * a.fold_per_block;
+    const uint64_t f_hi = f_lo + a.fold_per_block < words ? f_lo + a.fold_per_block : words;
+    const uint64_t f0 = f_lo + tid, f1 = f0 + nthreads;
+    int64_t fa0 = 0, fe0 = 0, fa1 = 0, fe1 = 0;
+    if (f0 < f_hi) {
+        fa0 = a.a_cur[f0];
+        fe0 = a.e_prev[f0];
+    }
+    if (f1 < f_hi) {
+        fa1 = a.a_cur[f1];
+        fe1 = a.e_prev[f1];
+    }
+
+    for (int i = tid; i < T * T; i += nthreads) sJ[i] = tab->J[i / T][i % T];
+    for (int i = tid; i < T; i += nthreads) {
+        sP[4 * i + 0] = tab->lam_v[i];
+        sP[4 * i + 1] = tab->tgt_v[i];
+        sP[4 * i + 2] = tab->lam_s[i];
+        sP[4 * i + 3] = tab->tgt_s[i];
+    }
+    for (int i = tid; i < (int)a.n_thr; i += nthreads) sThr[i] = tab->thr[i];
+    if (tid < a.n_off) {
+        sOff[tid] = a.off[tid][0] + a.off[tid][1] * LX + a.off[tid][2] * LX * LY;
+        sOff[32 + tid] = (a.off[tid][0] + 1) | ((a.off[tid][1] + 1) << 2) | ((a.off[tid][2] + 1) << 4);
+    }
+    stage_store(0, lv0, av0);
+    // a tile larger than kStage loads per lane stages the rest in turn
+    for (int base = kStage * nthreads; base < staged; base += kStage * nthreads) {
+        uint32_t lv[kStage];
+        float av[kStage];
+        stage_load(base, lv, av);
+        stage_store(base, lv, av);
     }
     const int wave = tid >> 6, lane = tid & 63, nwaves = nthreads >> 6;
     __syncthreads();
@@ -201,11 +220,70 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &a, const int blk) {
         const uint32_t old = sLab[tl];
         const uint32_t nw = sLab[sl];
         const uint32_t oid = label_id(old), nid = label_id(nw);
+        const uint32_t eto = label_type(old), etn = label_type(nw);
         const int cw = a.cell_words;
-        int64_t pv_old = 0, pv_new = 0;
-        if (oid != 0) pv_old = a.a_cur[(size_t)oid * cw] + a.e_prev[(size_t)oid * cw];
-        if (nid != 0) pv_new = a.a_cur[(size_t)nid * cw] + a.e_prev[(size_t)nid * cw];
-        bool go = old != nw && !(oid != 0 && unpack_vol(pv_old) <= 1);
+        // every gather the decision can need goes out before the first wait
+        // on one: both records; then, in a plane, for a target whose label
+        // would change, the chemotactic fields at target and source with
+        // their coefficients, and the moment words, anchors and table entries
+        // the length term reads. (A volume's terms, which would hold too many
+        // registers, and the nematic term gather once the records have decided
+        // the copy is priced.)
+        int64_t ra_o = 0, re_o = 0, ra_n = 0, re_n = 0;
+        if (oid != 0) {
+            ra_o = a.a_cur[(size_t)oid * cw];
+            re_o = a.e_prev[(size_t)oid * cw];
+        }
+        if (nid != 0) {
+            ra_n = a.a_cur[(size_t)nid * cw];
+            re_n = a.e_prev[(size_t)nid * cw];
+        }
+        const bool cand = old != nw;
+        int sx = 0, sy = 0, sz = 0;
+        if (FULL) {
+            sx = wrap_coord(gx + pdx, a.w);
+            sy = wrap_coord(gy + pdy, a.h);
+            sz = D3 ? wrap_coord(gz + pdz, a.d) : 0;
+        }
+        float chi_p[kMaxSpecies], ft_p[kMaxSpecies], fs_p[kMaxSpecies];
+        const bool chem = FULL && !D3 && (a.flags & kChemotaxis) && cand && nid != 0;
+        if (chem) {
+            const size_t ssite = ((size_t)sz * a.h + sy) * (size_t)a.w + sx;
+#pragma unroll
+            for (int q = 0; q < kMaxSpecies; ++q) {
+                if (q < a.n_species) {
+                    chi_p[q] = tab->chi[etn][q];
+                    ft_p[q] = a.fields[q][site];
+                    fs_p[q] = a.fields[q][ssite];
+                }
+            }
+        }
+        // planar length: words sx, sy, sxx, syy, sxy (moments 0, 1, 3, 4, 6)
+        constexpr int kPlanarWords[5] = {0, 1, 3, 4, 6};
+        int64_t lm_a[2][5], lm_e[2][5];
+        int32_t anc_p[2][2];
+        double lam_p[2], tgt_p[2];
+        const bool plen = FULL && !D3 && (a.flags & kLength) && cand;
+        if (plen) {
+#pragma unroll
+            for (int side = 0; side < 2; ++side) {
+                const uint32_t id = side == 0 ? oid : nid;
+                const uint32_t ty = side == 0 ? eto : etn;
+                lam_p[side] = tab->lam_len[ty];
+                tgt_p[side] = tab->tgt_len[ty];
+                if (id == 0) continue;
+                const size_t base = (size_t)id * cw + 1;
+#pragma unroll
+                for (int q = 0; q < 5; ++q) {
+                    lm_a[side][q] = a.a_cur[base + kPlanarWords[q]];
+                    lm_e[side][q] = a.e_prev[base + kPlanarWords[q]];
+                }
+                anc_p[side][0] = a.anchor[3 * id];
+                anc_p[side][1] = a.anchor[3 * id + 1];
+            }
+        }
+        const int64_t pv_old = ra_o + re_o, pv_new = ra_n + re_n;
+        bool go = cand && !(oid != 0 && unpack_vol(pv_old) <= 1);
 
         if (FULL && go && oid != 0 && (a.flags & kConnected) && tab->connected[label_type(old)]) {
             uint32_t present = 0u;
@@ -219,7 +297,6 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &a, const int blk) {
         }
 
         if (go) {
-            const uint32_t eto = label_type(old), etn = label_type(nw);
             float cn = 0.0f, co = 0.0f;
             int like_old = 0, like_new = 0;
             const float *rowN = sJ + etn * T;
@@ -276,13 +353,31 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &a, const int blk) {
                     dh = dh + ls * (dsf * (2.0f * (s - sP[4 * etn + 3]) + dsf));
                 }
             }
-            int sx = 0, sy = 0, sz = 0;
-            if (FULL) {
-                sx = wrap_coord(gx + pdx, a.w);
-                sy = wrap_coord(gy + pdy, a.h);
-                sz = D3 ? wrap_coord(gz + pdz, a.d) : 0;
+            if (FULL && !D3 && (a.flags & kLength)) {
+                for (int side = 0; side < 2; ++side) {
+                    const uint32_t id = side == 0 ? oid : nid;
+                    if (id == 0 || lam_p[side] == 0.0) continue;
+                    const int sign = side == 0 ? -1 : 1;
+                    int64_t m[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, after[9];
+#pragma unroll
+                    for (int q = 0; q < 5; ++q) m[kPlanarWords[q]] = lm_a[side][q] + lm_e[side][q];
+                    const int64_t n = unpack_vol(side == 0 ? pv_old : pv_new);
+                    const int64_t ux = fold_int(gx - anc_p[side][0], a.w);
+                    const int64_t uy = fold_int(gy - anc_p[side][1], a.h);
+                    after[0] = m[0] + sign * ux;
+                    after[1] = m[1] + sign * uy;
+                    after[3] = m[3] + sign * ux * ux;
+                    after[4] = m[4] + sign * uy * uy;
+                    after[6] = m[6] + sign * ux * uy;
+                    after[2] = after[5] = after[7] = after[8] = 0;
+                    // the label's length in the pass's snapshot, from the
+                    // words just gathered (the closed form of a plane)
+                    const double before = length_of(m, n, false) - tgt_p[side];
+                    const double now = length_of(after, n + sign, false) - tgt_p[side];
+                    dh = dh + (float)(lam_p[side] * (now * now - before * before));
+                }
             }
-            if (FULL && (a.flags & kLength)) {
+            if (FULL && D3 && (a.flags & kLength)) {
                 const bool three = D3;
                 for (int side = 0; side < 2; ++side) {
                     const uint32_t id = side == 0 ? oid : nid;
@@ -305,10 +400,9 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &a, const int blk) {
                     after[6] = m[6] + sign * ux * uy;
                     after[7] = m[7] + sign * ux * uz;
                     after[8] = m[8] + sign * uy * uz;
-                    // the label's length in the pass's snapshot: in a plane the
-                    // closed form from the words just gathered, in a volume (an
-                    // iterative eigensolver) once per label by the prepass
-                    double before = D3 ? a.len_before[id] : length_of(m, n, false) - tab->tgt_len[ty];
+                    // the label's length in the pass's snapshot, an iterative
+                    // eigensolver in a volume, once per label by the prepass
+                    double before = a.len_before[id];
                     double now = length_of(after, n + sign, three) - tab->tgt_len[ty];
                     dh = dh + (float)(tab->lam_len[ty] * (now * now - before * before));
                 }
@@ -349,12 +443,19 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &a, const int blk) {
                     dh = dh + (float)(-tab->lam_nem[ty] * ((now - before) * kNematicUnit));
                 }
             }
-            if (FULL && (a.flags & kChemotaxis) && nid != 0) {
+            if (chem) {
+#pragma unroll
+                for (int q = 0; q < kMaxSpecies; ++q) {
+                    if (q >= a.n_species || chi_p[q] == 0.0f) continue;
+                    dh = dh - chi_p[q] * (ft_p[q] - fs_p[q]);
+                }
+            }
+            if (FULL && D3 && (a.flags & kChemotaxis) && nid != 0) {
                 const size_t ssite = ((size_t)sz * a.h + sy) * (size_t)a.w + sx;
-                for (int s = 0; s < a.n_species; ++s) {
-                    float chi = tab->chi[etn][s];
+                for (int q = 0; q < a.n_species; ++q) {
+                    const float chi = tab->chi[etn][q];
                     if (chi == 0.0f) continue;
-                    dh = dh - chi * (a.fields[s][site] - a.fields[s][ssite]);
+                    dh = dh - chi * (a.fields[q][site] - a.fields[q][ssite]);
                 }
             }
             if (FULL && (a.flags & kMotility) && nid != 0 && tab->coef_act[etn] != 0.0f &&
@@ -451,6 +552,20 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &a, const int blk) {
                 }
             }
         }
+    }
+
+    // ---- the fold's stores, then the rest of its share
+    if (f0 < f_hi) {
+        a.a_next[f0] = fa0 + fe0;
+        a.g_zero[f0] = 0;
+    }
+    if (f1 < f_hi) {
+        a.a_next[f1] = fa1 + fe1;
+        a.g_zero[f1] = 0;
+    }
+    for (uint64_t i = f1 + nthreads; i < f_hi; i += nthreads) {
+        a.a_next[i] = a.a_cur[i] + a.e_prev[i];
+        a.g_zero[i] = 0;
     }
 
     // ---- acceptance count: one atomic per wavefront into a spread slot.
