@@ -288,32 +288,54 @@ __device__ __forceinline__ void check_recentre_body(const CellArgs &a, const int
 __device__ __forceinline__ void plan_body(const CellArgs &a, int64_t *e, const int lb, const int gb) {
     uint32_t l = lb * blockDim.x + threadIdx.x;
     if (l < 4 && l != 2 && a.flags_next) a.flags_next[l] = 0u;
-    uint32_t n = *a.n_labels;
+    // two rounds of loads: the label count with this label's type, then
+    // (for a live id) its record, the last pass's deltas and the type's
+    // division volume and death rate together; the record is brought
+    // current in registers
+    if (l >= a.capacity) return;
+    const uint32_t n = *a.n_labels;
+    const uint32_t tw = a.ltype[l];
     if (l >= n) return;
+    constexpr int kWords = 12;  // the widest record: volume/surface, nine moments, two field sums
+    const size_t base = (size_t)l * a.cw;
+    int64_t rec[kWords], del[kWords];
+#pragma unroll
+    for (int q = 0; q < kWords; ++q) {
+        rec[q] = 0;
+        del[q] = 0;
+        if (q < a.cw) {
+            rec[q] = a.state[base + q];
+            if (e) del[q] = e[base + q];
+        }
+    }
+    const uint32_t t = label_type(tw);
+    const DeviceTables *tab = a.tables;
+    const double div_vol = tab->div_vol[t], death = tab->death[t];
+#pragma unroll
+    for (int q = 0; q < kWords; ++q) rec[q] += del[q];
     if (e) {
-        for (int q = 0; q < a.cw; ++q) {
-            const size_t w = (size_t)l * a.cw + q;
-            a.state[w] += e[w];
-            e[w] = 0;
+#pragma unroll
+        for (int q = 0; q < kWords; ++q) {
+            if (q < a.cw) {
+                a.state[base + q] = rec[q];
+                e[base + q] = 0;
+            }
         }
     }
     a.plan[l] = 0;
     if (l == 0) return;
-    const int64_t *s = a.state + (size_t)l * a.cw;
-    int64_t v = unpack_vol(s[0]);
+    int64_t v = unpack_vol(rec[0]);
     if (v <= 0) return;
-    if (drifted(a, s, v)) a.flags[1] = 1;
-    uint32_t t = label_type(a.ltype[l]);
-    const DeviceTables *tab = a.tables;
-    if (tab->div_vol[t] > 0.0 && (double)v >= tab->div_vol[t] && v >= 4) {
+    if (drifted(a, rec, v)) a.flags[1] = 1;
+    if (div_vol > 0.0 && (double)v >= div_vol && v >= 4) {
         a.plan[l] = 1;
         double *g = a.geo + 6 * (size_t)l;
-        for (int ax = 0; ax < 3; ++ax) g[ax] = (double)s[1 + ax] / (double)v;
-        cov_largest(s + 1, v, a.d > 1, g + 3);
+        for (int ax = 0; ax < 3; ++ax) g[ax] = (double)rec[1 + ax] / (double)v;
+        cov_largest(rec + 1, v, a.d > 1, g + 3);
         a.flags[0] = 1;
-    } else if (tab->death[t] > 0.0) {
+    } else if (death > 0.0) {
         uint32_t hsh = hash32(hash32(l ^ a.key) ^ a.key2);
-        if ((double)hsh * (1.0 / 4294967296.0) < tab->death[t]) {
+        if ((double)hsh * (1.0 / 4294967296.0) < death) {
             a.plan[l] = 2;
             a.flags[0] = 1;
         }
