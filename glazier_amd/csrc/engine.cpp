@@ -233,6 +233,10 @@ PlainParams plain_params(const ghip_sim *s) {
     // with both contact rows flat across the medium and the cell type, a
     // medium neighbour prices like a cell one and the medium count drops out
     pp.needs_medium = !(pp.j[0][0] == pp.j[0][1] && pp.j[1][0] == pp.j[1][1]);
+    // a block's tile row by a multiply (exact while tiles_x * blocks < 2^32,
+    // which plain2d_tile ensures); one tile column needs none
+    const uint64_t tx = (uint64_t)std::max(1, s->tiles_x);
+    pp.tx_magic = tx == 1 ? 0u : (uint32_t)((((uint64_t)1 << 32) + tx - 1) / tx);
     return pp;
 }
 

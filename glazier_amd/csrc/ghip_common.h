@@ -49,6 +49,25 @@ GHD uint32_t second_key(uint32_t key) { return hash32(key ^ 0x85EBCA6BU); }
 GHD uint32_t site_pick(uint32_t key, uint32_t site) { return hash32(site ^ key); }
 GHD uint32_t site_draw(uint32_t key2, uint32_t p) { return hash32(p ^ key2); }
 
+// The pick among eight neighbours, site_pick(key, site) & 7, with the second
+// multiply on 24 bits. The pick reads bits 0-2 and 16-18 of that product,
+// and bit k of a product depends only on bits 0..k of its operands, so the
+// result is the same. On the device it is one full-rate v_mul_u32_u24 (the
+// compiler widens a masked multiply back to a 32-bit one, whose low bits are
+// the same, so the instruction is named).
+GHD uint32_t site_pick8(uint32_t key, uint32_t site) {
+    uint32_t x = site ^ key;
+    x ^= x >> 16;
+    x *= 0x7feb352dU;
+    x ^= x >> 15;
+#if defined(__HIP_DEVICE_COMPILE__)
+    asm("v_mul_u32_u24_e32 %0, %1, %2" : "=v"(x) : "s"(0x846ca68bU & 0xFFFFFFu), "v"(x));
+#else
+    x = (x & 0xFFFFFFu) * (0x846ca68bU & 0xFFFFFFu);
+#endif
+    return (x ^ (x >> 16)) & 7u;
+}
+
 // Second 24-bit uniform, used when the neighbour count is not a power of two
 // and the pick has consumed the whole first draw.
 GHD uint32_t second_uniform(uint32_t h) { return hash32(h ^ 0x68E31DA4U) >> 8; }

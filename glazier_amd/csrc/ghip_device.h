@@ -121,6 +121,8 @@ struct PlainParams {
     const uint32_t *thr;    // integral acceptance table
     uint32_t n_thr;
     uint32_t needs_medium;  // whether a medium neighbour can change a price
+    uint32_t tx_magic;      // ceil(2^32 / tiles_x): b / tiles_x = umulhi(b, tx_magic) while
+                            // tiles_x * blocks < 2^32 (the tile choice ensures it; 0 when tiles_x is 1)
 };
 
 // A small plain lattice stepped whole in one workgroup (sweep_plain.hip):

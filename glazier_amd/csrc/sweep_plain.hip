@@ -89,8 +89,11 @@ __device__ __forceinline__ PlainCounts plain_counts(uint32_t old, uint32_t nw, u
     return c;
 }
 
+// `site_of()` yields the target's site; it is asked only when the acceptance
+// table decides.
+template <class SiteOf>
 __device__ __forceinline__ PlainMove plain_move_counts(const PlainParams &p, const uint32_t *thr_s, uint32_t key,
-                                                       uint32_t key2, uint32_t site, uint32_t old, uint32_t nw,
+                                                       uint32_t key2, SiteOf site_of, uint32_t old, uint32_t nw,
                                                        const PlainCounts &c, int64_t pv_old, int64_t pv_new) {
     const uint32_t oid = label_id(old), nid = label_id(nw);
     const int unlike_old = c.unlike_old, unlike_new = c.unlike_new;
@@ -101,8 +104,8 @@ __device__ __forceinline__ PlainMove plain_move_counts(const PlainParams &p, con
     // a copy that would empty a cell is refused (no branch before the
     // records are used, so no gather waits behind another)
     const bool emptying = oid != 0 && vol_old <= 1;
-    const int med = c.med;
     const int etn = nid != 0 ? 1 : 0, eto = oid != 0 ? 1 : 0;
+    const int med = c.med;
     // neighbours unlike the new label: 8 - like_new; the medium ones among
     // them are all medium neighbours when the new label is a cell, none
     // when it is the medium (then like_new counts them). So `med` enters
@@ -136,7 +139,7 @@ __device__ __forceinline__ PlainMove plain_move_counts(const PlainParams &p, con
         dh = dh + p.lam_s * (dsf * (2.0f * (s - p.tgt_s) + dsf));
     }
     bool acc = dh <= 0.0f;
-    if (!acc && dh < (float)p.n_thr) acc = (site_draw(key2, site_pick(key, site)) >> 8) < thr_s[(int)dh];
+    if (!acc && dh < (float)p.n_thr) acc = (site_draw(key2, site_pick(key, site_of())) >> 8) < thr_s[(int)dh];
     mv.acc = !emptying && acc;
     return mv;
 }
@@ -147,7 +150,7 @@ __device__ __forceinline__ PlainMove plain_move(const PlainParams &p, const uint
                                                 uint32_t n5, uint32_t n6, uint32_t n7, int64_t pv_old,
                                                 int64_t pv_new, bool has_medium) {
     const PlainCounts c = plain_counts(old, nw, n0, n1, n2, n3, n4, n5, n6, n7, has_medium);
-    return plain_move_counts(p, thr_s, key, key2, site, old, nw, c, pv_old, pv_new);
+    return plain_move_counts(p, thr_s, key, key2, [=] { return site; }, old, nw, c, pv_old, pv_new);
 }
 }  // namespace
 
@@ -169,13 +172,21 @@ struct PlainTile {
     static_assert((1 << kQBits) == kTargets && kQBits + 3 <= 16, "16-bit queue entries");
 
     const uint32_t *tile;
-    int x0, y0;
+    uint32_t site_base;  // site of the tile's target 0: (y0 + cy) w + x0 + cx
+    int tl_base;         // its LDS word: (1 + cy) kPitch + 4 + cx
+    uint32_t w2;         // two lattice rows, below 2^24 (the tile choice ensures it)
 
-    // target q of the tile: row j (uniform in a wavefront), column i
-    __device__ __forceinline__ void target(const SweepArgs &a, int q, uint32_t *site, int *tl) const {
-        const int j = q / kTRow, i = q % kTRow;
-        *site = (uint32_t)(y0 + 2 * j + a.cy) * (uint32_t)a.w + (uint32_t)(x0 + a.cx + 2 * i);
-        *tl = (1 + 2 * j + a.cy) * kPitch + 4 + a.cx + 2 * i;
+    // target q of the tile lies on target row j = q / kTRow (uniform in a
+    // wavefront), column i = q % kTRow: its LDS word, and its site, which
+    // pricing asks for only when it needs it. Row steps are 24-bit
+    // multiplies (j < 64).
+    __device__ __forceinline__ int target_tl(uint32_t q) const {
+        const uint32_t j = q / kTRow, i = q % kTRow;
+        return tl_base + (int)(j * (2u * kPitch) + 2u * i);
+    }
+    __device__ __forceinline__ uint32_t target_site(uint32_t q) const {
+        const uint32_t j = q / kTRow, i = q % kTRow;
+        return site_base + __umul24(j, w2) + 2u * i;
     }
     // the neighbour pick k (0..7) names, in ORDER3's planar order, (dx, dy)
     // row by row; dx + 1 and dy + 1 are the k-th 2-bit fields of 0x9224 and
@@ -197,15 +208,14 @@ struct PlainTile {
     __device__ __forceinline__ void draw(const SweepArgs &a, uint32_t &moves, uint32_t &picks) const {
         static_assert(NT % kTRow == 0, "a lane keeps its column");
         constexpr int kJStep = NT / kTRow;
-        const int jl = (int)threadIdx.x / kTRow, il = (int)threadIdx.x % kTRow;
-        const uint32_t site0 = (uint32_t)(y0 + 2 * jl + a.cy) * (uint32_t)a.w + (uint32_t)(x0 + a.cx + 2 * il);
-        const int tl0 = (1 + 2 * jl + a.cy) * kPitch + 4 + a.cx + 2 * il;
-        const uint32_t dsite = 2u * (uint32_t)kJStep * (uint32_t)a.w;
+        const uint32_t site0 = target_site(threadIdx.x);
+        const int tl0 = target_tl(threadIdx.x);
+        const uint32_t dsite = (uint32_t)kJStep * w2;
 #pragma unroll
         for (int k = K0; k < K1; ++k) {
             const uint32_t site = site0 + (uint32_t)k * dsite;
             const int tl = tl0 + k * 2 * kJStep * kPitch;
-            const uint32_t pk = site_pick(a.key, site) & 7u;
+            const uint32_t pk = site_pick8(a.key, site);
             moves |= (tile[tl] != picked(pk, tl) ? 1u : 0u) << k;
             picks |= pk << (3 * k);
         }
@@ -239,58 +249,58 @@ struct PlainTile {
     }
 
     // Phase B: price the nq queued targets, write the accepted copies'
-    // labels, and return this lane's accepted count. It redraws each
-    // target's pick round (cheaper than holding it in LDS) and issues the
-    // per-label gathers together. The record changes of a lane's first
-    // entry (queue entry tid) leave in
-    // d_old and d_new as id | (ds + 8) / 2 << 28 (the surface change ds is
-    // even, -8..8; the volume changes by -1 and +1), or kEmpty, for
-    // merge_deltas when the launch merges; those of later entries (a tile
-    // with more than NT queued targets), and all of them when it does not,
-    // go straight to the records.
+    // labels, and return this lane's accepted count. Each entry carries its
+    // pick, the target's neighbourhood is read from LDS first and the
+    // per-label gathers go out together behind it. Records are one word per
+    // label here (the plain pass keeps no moments), so a label's record is
+    // at its id; with e_zero (the previous pass accepted nothing anywhere,
+    // a uniform fact taken outside the loop so the loop body has no branch
+    // before its decision) the previous pass's deltas are all zero and not
+    // read. The record changes of a lane's first entry (queue entry tid)
+    // leave in d_old and d_new as id | (ds + 8) / 2 << 28 (the surface
+    // change ds is even, -8..8; the volume changes by -1 and +1), or kEmpty,
+    // for merge_deltas when the launch merges; those of later entries (a
+    // tile with more than NT queued targets), and all of them when it does
+    // not, go straight to the records.
     static constexpr uint32_t kEmpty = 0xFFFFFFFFu;
 
-    template <int NT>
+    template <int NT, bool e_zero>
     __device__ __forceinline__ uint32_t phase_b(const SweepArgs &a, const PlainParams &p, const uint32_t *thr_s,
-                                                const uint16_t *queue, uint32_t nq, bool has_medium, bool e_zero,
+                                                const uint16_t *queue, uint32_t nq, bool has_medium,
                                                 uint32_t &d_old, uint32_t &d_new) const {
-        const uint32_t cw = (uint32_t)a.cell_words;
         uint32_t my_acc = 0;
         d_old = d_new = kEmpty;
         for (uint32_t qi = threadIdx.x; qi < nq; qi += NT) {
-            uint32_t site;
-            int tl;
             const uint32_t entry = queue[qi];
-            target(a, (int)(entry & (uint32_t)(kTargets - 1)), &site, &tl);
+            const uint32_t q = entry & (uint32_t)(kTargets - 1);
+            const int tl = target_tl(q);
             const uint32_t old = tile[tl];
             const uint32_t nw = picked(entry >> kQBits, tl);
             const uint32_t oid = label_id(old), nid = label_id(nw);
-            // the gathers go out together, before anything waits on them;
-            // label 0's record is real memory whose value is never used. When
-            // the previous pass accepted nothing its deltas are all zero.
-            const int64_t ao = a.a_cur[oid * cw], an = a.a_cur[nid * cw];
-            int64_t eo = 0, en = 0;
-            if (!e_zero) {
-                eo = a.e_prev[oid * cw];
-                en = a.e_prev[nid * cw];
-            }
             const uint32_t n0 = tile[tl - kPitch - 1], n1 = tile[tl - kPitch], n2 = tile[tl - kPitch + 1];
             const uint32_t n3 = tile[tl - 1], n4 = tile[tl + 1];
             const uint32_t n5 = tile[tl + kPitch - 1], n6 = tile[tl + kPitch], n7 = tile[tl + kPitch + 1];
-            const PlainMove mv =
-                plain_move(p, thr_s, a.key, a.key2, site, old, nw, n0, n1, n2, n3, n4, n5, n6, n7, ao + eo,
-                           an + en, has_medium);
+            // label 0's record is real memory whose value is never used
+            const int64_t ao = a.a_cur[oid], an = a.a_cur[nid];
+            int64_t eo = 0, en = 0;
+            if (!e_zero) {
+                eo = a.e_prev[oid];
+                en = a.e_prev[nid];
+            }
+            const PlainCounts c = plain_counts(old, nw, n0, n1, n2, n3, n4, n5, n6, n7, has_medium);
+            const PlainMove mv = plain_move_counts(
+                p, thr_s, a.key, a.key2, [&] { return target_site(q); }, old, nw, c, ao + eo, an + en);
             if (!mv.acc) continue;
-            a.labels[site] = nw;
+            a.labels[target_site(q)] = nw;
             if (a.merge_deltas && qi < (uint32_t)NT) {
                 if (oid != 0) d_old = oid | (uint32_t)((mv.ds_old + 8) >> 1) << kTypeShift;
                 if (nid != 0) d_new = nid | (uint32_t)((mv.ds_new + 8) >> 1) << kTypeShift;
             } else {
                 if (oid != 0)
-                    atomicAdd(reinterpret_cast<unsigned long long *>(a.f_acc + oid * cw),
+                    atomicAdd(reinterpret_cast<unsigned long long *>(a.f_acc + oid),
                               (unsigned long long)pack_vs(-1, mv.ds_old));
                 if (nid != 0)
-                    atomicAdd(reinterpret_cast<unsigned long long *>(a.f_acc + nid * cw),
+                    atomicAdd(reinterpret_cast<unsigned long long *>(a.f_acc + nid),
                               (unsigned long long)pack_vs(1, mv.ds_new));
             }
             ++my_acc;
@@ -313,7 +323,6 @@ struct PlainTile {
     __device__ __forceinline__ void merge_deltas(const SweepArgs &a, uint16_t *queue, uint32_t d_old,
                                                  uint32_t d_new) const {
         const int tid = threadIdx.x;
-        const uint32_t cw = (uint32_t)a.cell_words;
         unsigned long long *vals = reinterpret_cast<unsigned long long *>(queue);
         uint32_t *keys = reinterpret_cast<uint32_t *>(vals + kSlots);
         for (int i = tid; i < kSlots; i += NT) {
@@ -335,14 +344,14 @@ struct PlainTile {
                 }
                 ++h;
             }
-            atomicAdd(reinterpret_cast<unsigned long long *>(a.f_acc + id * cw), (unsigned long long)d);
+            atomicAdd(reinterpret_cast<unsigned long long *>(a.f_acc + id), (unsigned long long)d);
         };
         if (d_old != kEmpty) add(d_old, -1);
         if (d_new != kEmpty) add(d_new, 1);
         __syncthreads();
         for (int i = tid; i < kSlots; i += NT) {
             const uint32_t id = keys[i];
-            if (id != kEmpty) atomicAdd(reinterpret_cast<unsigned long long *>(a.f_acc + id * cw), vals[i]);
+            if (id != kEmpty) atomicAdd(reinterpret_cast<unsigned long long *>(a.f_acc + id), vals[i]);
         }
     }
 };
@@ -362,11 +371,17 @@ __global__ __launch_bounds__(NT) void sweep_plain2d_kernel(SweepArgs a, PlainPar
     __shared__ uint32_t acc_block;
 
     const int tid = threadIdx.x;
-    const int b = pass_tile(blockIdx.x, a.xcd_chunk, a.reverse, a.tiles_x * a.tiles_y);
-    const int tile_x = b % a.tiles_x;
-    const int tile_y = b / a.tiles_x;
-    const int x0 = tile_x * PX, y0 = tile_y * PY;
+    const uint32_t b = (uint32_t)pass_tile(blockIdx.x, a.xcd_chunk, a.reverse, a.tiles_x * a.tiles_y);
+    const uint32_t tile_y = a.tiles_x == 1 ? b : __umulhi(b, p.tx_magic);
+    const uint32_t tile_x = b - tile_y * (uint32_t)a.tiles_x;
+    const int x0 = (int)tile_x * PX, y0 = (int)tile_y * PY;
     const int W = a.w, H = a.h;
+    // row offsets of the tile's halo rows (each wraps at the lattice's edge)
+    // and of the row above the tile (wrapping in unsigned arithmetic when y0
+    // is 0; every row it is used for lies below it)
+    const uint32_t off_top = (uint32_t)(y0 == 0 ? H - 1 : y0 - 1) * (uint32_t)W;
+    const uint32_t off_bottom = (uint32_t)(y0 + PY >= H ? 0 : y0 + PY) * (uint32_t)W;
+    const uint32_t off_above = (uint32_t)(y0 - 1) * (uint32_t)W;
 
     if (tid == 0) {
         qcount = 0;
@@ -381,24 +396,27 @@ __global__ __launch_bounds__(NT) void sweep_plain2d_kernel(SweepArgs a, PlainPar
     if (a.pass_flags && blockIdx.x == 0 && tid == 0) a.pass_flags[(a.pass_slot + 1) & 3u] = 0;
 
     // ---- issue every global read of the workgroup before the first wait, in
-    // the order they are used: the halo words and the acceptance table, the
-    // tile's rows, the fold's words, the pass flags. Every lane issues every
+    // the order they are used: the halo words, the acceptance table and the
+    // pass flags, the tile's rows, the fold's words. Every lane issues every
     // load, with no branch (a lane with nothing to load reads a valid word
     // and ignores it), so each wait covers exactly the loads it needs: a
     // wavefront's wait for a load is a wait for every load issued before it.
-    const uint32_t n_labels = *a.n_labels;
-    const uint32_t words = a.fold_words ? a.fold_words : n_labels * (uint32_t)a.cell_words;
+    // the fold's extent: from the host when the label count is fixed, else the
+    // device count (read only then, so no load ahead of the rows waits on it)
+    uint32_t words = a.fold_words;
+    if (words == 0) words = *a.n_labels;
     const bool h_live = tid < kRows * 2;
     uint32_t hv;
     {
-        const int r = h_live ? tid >> 1 : 0, right = tid & 1;
-        int gy = y0 - 1 + r;
-        gy += gy < 0 ? H : 0;
-        gy -= gy >= H ? H : 0;
-        int gx = right ? x0 + PX : x0 - 1;
-        gx += gx < 0 ? W : 0;
-        gx -= gx >= W ? W : 0;
-        hv = a.labels[(uint32_t)gy * (uint32_t)W + (uint32_t)gx];
+        // lane 2 r + side reads row r - 1 of the tile, the column left of it
+        // (side 0) or right of it (side 1)
+        const uint32_t r = h_live ? (uint32_t)tid >> 1 : 0u;
+        const uint32_t x_left = (uint32_t)(x0 == 0 ? W - 1 : x0 - 1);
+        const uint32_t x_right = (uint32_t)(x0 + PX >= W ? 0 : x0 + PX);
+        const uint32_t gx = (tid & 1) ? x_right : x_left;
+        const uint32_t off = r == 0 ? off_top : r >= (uint32_t)kRows - 1 ? off_bottom
+                                                                         : off_above + __umul24(r, (uint32_t)W);
+        hv = a.labels[off + gx];
     }
     // a lane with no halo word stores its load into a spare word past the
     // tile, so the load has an unconditional use and stays where it is
@@ -407,6 +425,11 @@ __global__ __launch_bounds__(NT) void sweep_plain2d_kernel(SweepArgs a, PlainPar
     constexpr int kSpare = kRows * kPitch + 1;
     const int hpos = h_live ? (tid >> 1) * kPitch + ((tid & 1) ? 4 + PX : 3) : kSpare + (tid & 1);
     const uint32_t tv = p.thr[tid < (int)p.n_thr ? tid : 0];
+    // the pass flags of passes p - 1 and p - 2 (read first of the uniform
+    // words: the decisions on the fold and the deltas wait on them)
+    const uint32_t *flags = a.pass_flags ? a.pass_flags : reinterpret_cast<const uint32_t *>(a.accepted);
+    const uint32_t f1 = flags[(a.pass_slot + 3) & 3u], f2 = flags[(a.pass_slot + 2) & 3u];
+    __builtin_amdgcn_sched_barrier(1);
     // load k of a lane reads tile row r = k kRowsPerLoad + tid / kC4 (row 0
     // is the halo row above, kRows - 1 the one below, each wrapping at the
     // lattice's edge; rows past the tile reread the last one), column 4 (tid
@@ -415,9 +438,7 @@ __global__ __launch_bounds__(NT) void sweep_plain2d_kernel(SweepArgs a, PlainPar
     static_assert(NT % kC4 == 0, "a load instruction covers whole rows");
     const int c4 = tid % kC4, rl = tid / kC4;
     const uint32_t *col = a.labels + (uint32_t)x0 + 4u * (uint32_t)c4;
-    const uint32_t off_top = (uint32_t)(y0 == 0 ? H - 1 : y0 - 1) * (uint32_t)W;
-    const uint32_t off_bottom = (uint32_t)(y0 + PY >= H ? 0 : y0 + PY) * (uint32_t)W;
-    const uint32_t off_lane = (uint32_t)(y0 - 1 + rl) * (uint32_t)W;
+    const uint32_t off_lane = off_above + __umul24((uint32_t)rl, (uint32_t)W);
     const uint32_t row_step = (uint32_t)kRowsPerLoad * (uint32_t)W;
     uint4 v[kLoads];
     int vdst[kLoads];
@@ -428,18 +449,15 @@ __global__ __launch_bounds__(NT) void sweep_plain2d_kernel(SweepArgs a, PlainPar
         v[k] = *reinterpret_cast<const uint4 *>(col + off);
         vdst[k] = r < kRows ? r * kPitch + 4 + 4 * c4 : -1;
     }
+    // the global loads stay in this order (the compiler may not move one
+    // past these barriers): a later load scheduled ahead of a row would make
+    // that row's wait cover it
+    __builtin_amdgcn_sched_barrier(1);
     const uint32_t flo = blockIdx.x * a.fold_per_block;
     const uint32_t fhi = min(flo + a.fold_per_block, words);
     const uint32_t fi = flo + tid;
     const uint32_t fc = fi < fhi ? fi : 0u;
     const int64_t fa = a.a_cur[fc], fe = a.e_prev[fc];
-    // Passes p - 1 and p - 2 accepted nothing anywhere: the previous pass's
-    // deltas are all zero (e_zero), and the fold is the identity as well
-    // (fold_idle): the record buffer it would write already holds the
-    // snapshot, and the delta buffer it would clear is already clear. Pass p
-    // cleared the word of pass p + 1 above, which no pass p reads.
-    const uint32_t *flags = a.pass_flags ? a.pass_flags : reinterpret_cast<const uint32_t *>(a.accepted);
-    const uint32_t f1 = flags[(a.pass_slot + 3) & 3u], f2 = flags[(a.pass_slot + 2) & 3u];
 
     // ---- land them in two parts. The first kL1 loads hold the tile's first
     // rows; the lane's first kA1 targets read no row beyond them, so they are
@@ -463,7 +481,9 @@ __global__ __launch_bounds__(NT) void sweep_plain2d_kernel(SweepArgs a, PlainPar
             if (p.needs_medium) lo = min(lo, min(min(v[k].x, v[k].y), min(v[k].z, v[k].w)));
         }
     };
-    const PlainTile<PX, PY> pt{tile, x0, y0};
+    const int cx = a.cx & 1, cy = a.cy & 1;
+    const PlainTile<PX, PY> pt{tile, (uint32_t)(y0 + cy) * (uint32_t)W + (uint32_t)(x0 + cx),
+                               (1 + cy) * kPitch + 4 + cx, 2u * (uint32_t)W};
     uint32_t moves = 0, picks = 0;
     land(0, kA1 > 0 ? kL1 : kLoads);
     tile[hpos] = hv;
@@ -494,7 +514,9 @@ __global__ __launch_bounds__(NT) void sweep_plain2d_kernel(SweepArgs a, PlainPar
     pt.template queue_moves<NT>(moves, picks, queue, &qcount);
     __syncthreads();
     uint32_t d_old, d_new;
-    const uint32_t my_acc = pt.template phase_b<NT>(a, p, thr_s, queue, qcount, has_medium, e_zero, d_old, d_new);
+    const uint32_t nq = qcount;
+    const uint32_t my_acc = e_zero ? pt.template phase_b<NT, true>(a, p, thr_s, queue, nq, has_medium, d_old, d_new)
+                                   : pt.template phase_b<NT, false>(a, p, thr_s, queue, nq, has_medium, d_old, d_new);
     if (my_acc) atomicAdd(&acc_block, my_acc);
     __syncthreads();
     // every queue entry is read; a tile with accepted copies merges their
@@ -675,12 +697,20 @@ void plain2d_tile(int w, int h, int *px, int *py) {
     } else {
         *px = *py = 0;
     }
+    // the pass's 24-bit row multiplies need two rows below 2^24 sites, and
+    // its tile-row reciprocal tiles_x * blocks below 2^32
+    if (*px > 0) {
+        const uint64_t tx = (uint64_t)(w / *px), blocks = tx * (uint64_t)(h / *py);
+        if (w >= (1 << 23) || tx * blocks >= ((uint64_t)1 << 32)) *px = *py = 0;
+    }
 }
 
 hipError_t launch_sweep_plain2d(const SweepArgs &a, const PlainParams &p, int blocks,
                                 hipStream_t stream) {
     int px = 0, py = 0;
     plain2d_tile(a.w, a.h, &px, &py);
+    // one record word per label: the pass addresses a label's record by its id
+    if (a.cell_words != 1) return hipErrorInvalidValue;
     if (px == 256)
         hipLaunchKernelGGL((sweep_plain2d_kernel<256, 32, 512>), dim3(blocks), dim3(512), 0, stream, a, p);
     else if (px == 128 && py == 8)

@@ -1,7 +1,7 @@
 // Host build of arithmetic the kernels share with the host
 // (glazier_amd/csrc/ghip_common.h): the engine's own f32 exponential and
-// logarithm, and the volume's connectivity test, so the CPU tests can hold
-// them to the C library and to the oracle.
+// logarithm, the volume's connectivity test and the plain pass's 24-bit
+// pick, so the CPU tests can hold them to the C library and to the oracle.
 #include "../../glazier_amd/csrc/ghip_common.h"
 
 extern "C" {
@@ -10,6 +10,10 @@ void engine_exp2_f32(const float *x, float *y, long n) {
 }
 void engine_log2_f32(const float *x, float *y, long n) {
     for (long i = 0; i < n; ++i) y[i] = ghip::log2_f32(x[i]);
+}
+// The eight-neighbour pick of each (key, site) pair.
+void engine_pick8(const uint32_t *key, const uint32_t *site, uint32_t *out, long n) {
+    for (long i = 0; i < n; ++i) out[i] = ghip::site_pick8(key[i], site[i]);
 }
 // One byte per ring mask in [lo, hi): whether its members are one piece.
 void engine_ring_table(uint32_t lo, uint32_t hi, uint8_t *out) {

@@ -41,6 +41,8 @@ def engine_host(tmp_path_factory):
         getattr(lib, name).restype = None
     lib.engine_ring_table.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_void_p]
     lib.engine_ring_table.restype = None
+    lib.engine_pick8.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_long]
+    lib.engine_pick8.restype = None
     return lib
 
 
@@ -166,3 +168,36 @@ def test_every_volume_ring_mask_agrees_three_ways(engine_host):
         for b in range(a + 1, 26):
             near = max(abs(pos[a][i] - pos[b][i]) for i in range(3)) <= 1
             assert uf[(1 << a) | (1 << b)] == near, (a, b)
+
+
+def _lowbias32(x):
+    """Wellons' lowbias32 over a uint32 array, in 64-bit lanes."""
+    x = x.astype(np.uint64)
+    m = np.uint64(0xFFFFFFFF)
+    x ^= x >> np.uint64(16)
+    x = (x * np.uint64(0x7FEB352D)) & m
+    x ^= x >> np.uint64(15)
+    x = (x * np.uint64(0x846CA68B)) & m
+    x ^= x >> np.uint64(16)
+    return x.astype(np.uint32)
+
+
+def test_the_plain_pass_pick_is_the_hash_pick(engine_host):
+    """site_pick8 takes the hash's second multiply on 24 bits; its pick must
+    be the full hash's low three bits for every key and site. Random pairs,
+    then every site of a 4096^2 lattice's first rows under extreme keys."""
+    rng = np.random.default_rng(5)
+    key = rng.integers(0, 2**32, size=1 << 22, dtype=np.uint64).astype(np.uint32)
+    site = rng.integers(0, 2**32, size=1 << 22, dtype=np.uint64).astype(np.uint32)
+    rows = np.arange(1 << 20, dtype=np.uint32)
+    for k in (0, 1, 0x80000000, 0xFFFFFFFF, 0x9E3779B9):
+        key = np.concatenate([key, np.full(rows.size, k, dtype=np.uint32)])
+        site = np.concatenate([site, rows])
+    got = np.empty_like(key)
+    engine_host.engine_pick8(key.ctypes.data, site.ctypes.data, got.ctypes.data, key.size)
+    want = _lowbias32(key ^ site) & np.uint32(7)
+    assert np.array_equal(got, want)
+    # and the oracle's hash agrees with the numpy restatement used here
+    lib = oracle.library()
+    for i in range(64):
+        assert lib.oracle_site_pick(int(key[i]), int(site[i])) & 7 == int(want[i])
