@@ -222,29 +222,35 @@ struct PlainTile {
     }
 
     // The queue entries of the targets `moves` marks, each with its pick:
-    // one reservation per wavefront.
+    // one reservation per wavefront, its entries ordered by the lane's
+    // target k (one target row) and then by lane, so the pricing wavefront
+    // that takes 64 consecutive entries writes its accepted labels into one
+    // or two target rows.
     template <int NT>
     __device__ __forceinline__ void queue_moves(uint32_t moves, uint32_t picks, uint16_t *queue,
                                                 uint32_t *qcount) const {
         constexpr int kPerLane = kTargets / NT;
-        static_assert(kTargets % NT == 0 && kPerLane <= 7, "a lane's count fits three bits");
+        static_assert(kTargets % NT == 0 && kPerLane <= 8, "a lane's targets fit the move byte");
         const int tid = threadIdx.x;
-        const uint32_t cnt = (uint32_t)__builtin_popcount(moves);
-        const unsigned long long c0 = __ballot(cnt & 1u), c1 = __ballot(cnt & 2u), c2 = __ballot(cnt & 4u);
-        const uint32_t total = (uint32_t)(__builtin_popcountll(c0) + 2 * __builtin_popcountll(c1) +
-                                          4 * __builtin_popcountll(c2));
+        unsigned long long m[kPerLane];
+        uint32_t total = 0;
+#pragma unroll
+        for (int k = 0; k < kPerLane; ++k) {
+            m[k] = __ballot((moves >> k) & 1u);
+            total += (uint32_t)__builtin_popcountll(m[k]);
+        }
         if (total == 0) return;
-        auto below = [](unsigned long long m) {
-            return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+        auto below = [](unsigned long long x) {
+            return __builtin_amdgcn_mbcnt_hi((uint32_t)(x >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)x, 0u));
         };
         uint32_t base = 0;
         if ((tid & 63) == 0) base = atomicAdd(qcount, total);
         base = __builtin_amdgcn_readfirstlane(base);
-        uint32_t pos = base + below(c0) + 2u * below(c1) + 4u * below(c2);
 #pragma unroll
         for (int k = 0; k < kPerLane; ++k) {
             if (moves & (1u << k))
-                queue[pos++] = (uint16_t)((k * NT + tid) | (int)((picks >> (3 * k)) & 7u) << kQBits);
+                queue[base + below(m[k])] = (uint16_t)((k * NT + tid) | (int)((picks >> (3 * k)) & 7u) << kQBits);
+            base += (uint32_t)__builtin_popcountll(m[k]);
         }
     }
 
@@ -314,8 +320,12 @@ struct PlainTile {
     // delta); the copies add into it with LDS atomics, and then each label
     // the tile touched costs one global atomic instead of one per copy (a
     // label that finds no slot in a few probes goes straight to its record).
-    // Integer sums in any order: the records are exactly those of one atomic
-    // per copy. Called by the whole workgroup.
+    // The table keeps the 16 records of one 128-byte line in 16 adjacent
+    // slots (a line hashes to a group of 16, a taken slot probes the next
+    // group), so each wavefront's atomics land on a few lines: a global
+    // atomic instruction costs per line it touches, and a tile's labels are
+    // mostly consecutive ids. Integer sums in any order: the records are
+    // exactly those of one atomic per copy. Called by the whole workgroup.
     static constexpr int kSlots = kTargets >= 2048 ? 256 : kTargets >= 1024 ? 64 : 32;
     static_assert(kSlots * 12 <= kTargets * 2, "the table fits in the queue's bytes");
 
@@ -333,16 +343,16 @@ struct PlainTile {
         auto add = [&](uint32_t packed, int64_t dv) {
             const uint32_t id = label_id(packed);
             const int64_t d = pack_vs(dv, 2 * (int64_t)(packed >> kTypeShift) - 8);
-            uint32_t h = (id * 0x9E3779B1u) >> 24;
+            uint32_t h = ((((id >> 4) * 0x9E3779B1u) >> 24) & ~15u) | (id & 15u);
 #pragma unroll
-            for (int probe = 0; probe < 4; ++probe) {
+            for (int probe = 0; probe < 8; ++probe) {
                 h &= (uint32_t)(kSlots - 1);
                 const uint32_t k = atomicCAS(&keys[h], kEmpty, id);
                 if (k == kEmpty || k == id) {
                     atomicAdd(&vals[h], (unsigned long long)d);
                     return;
                 }
-                ++h;
+                h += 16;
             }
             atomicAdd(reinterpret_cast<unsigned long long *>(a.f_acc + id), (unsigned long long)d);
         };
