@@ -126,7 +126,165 @@ hipError_t launch_consolidate(int64_t *a, int64_t *e, const uint32_t *n_labels, 
     return hipGetLastError();
 }
 
+// ---- the recount of a plane whose records are volume and surface alone
+//
+// A workgroup of 256 lanes takes kPlaneRows rows of 256 columns. The rows
+// with a one-site halo land in LDS by 16-byte loads (a row's interior at word
+// 4 of its LDS row, the left halo word at 3, the right one past the
+// interior). Each lane walks its column down with the 3 x 3 neighbourhood in
+// registers (three LDS reads a row), summing volume and bonds of a run of one
+// label; where runs end, the wavefront first sums the runs of adjacent lanes
+// that share a label (a cell spans several columns), and only each group's
+// first lane adds into the workgroup's LDS table. The table keeps the 16
+// records of a 128-byte line in adjacent slots, so its global atomics land on
+// few lines per instruction. Integer sums: the result does not depend on
+// the order of any of it.
+constexpr int kPlaneRows = 16;
+constexpr int kPlaneCols = 256;
+constexpr int kPlanePitch = kPlaneCols + 8;
+constexpr int kPlaneSlots = 256;
+
+bool recount_plane_applies(const CellArgs &a) {
+    return a.d == 1 && a.cw == 1 && a.w % kPlaneCols == 0 && a.h % kPlaneRows == 0 && (a.n_off == 4 || a.n_off == 8);
+}
+
+// The sum of `v` over the lanes of this lane's group: maximal runs of
+// adjacent active lanes holding one key. Returns true on the group's first
+// lane, which then holds the group's sum in `sum`.
+__device__ __forceinline__ bool wave_group_sum(bool active, uint32_t key, uint32_t v, uint32_t *sum) {
+    const int lane = (int)(threadIdx.x & 63);
+    const uint32_t pkey = __shfl_up(key, 1, 64);
+    const bool pact = __shfl_up(active ? 1 : 0, 1, 64) != 0;
+    const bool head = active && (lane == 0 || !pact || pkey != key);
+    const unsigned long long heads = __ballot(head);
+    const unsigned long long breaks = heads | ~__ballot(active);
+    // inclusive prefix sum over the wavefront of the active values
+    const uint32_t own = active ? v : 0u;
+    uint32_t p = own;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t q = __shfl_up(p, (unsigned)d, 64);
+        if (lane >= d) p += q;
+    }
+    // a group ends before the next break after its head (or at lane 63)
+    const unsigned long long later = lane == 63 ? 0ull : (breaks >> (lane + 1));
+    const int end = later ? lane + __builtin_ctzll(later) : 63;
+    *sum = __shfl(p, end, 64) - (p - own);
+    return head;
+}
+
+template <int NOFF>
+__global__ __launch_bounds__(kPlaneCols) void recount_plane_kernel(CellArgs a) {
+    __shared__ __attribute__((aligned(16))) uint32_t rt[(kPlaneRows + 2) * kPlanePitch];
+    __shared__ uint32_t keys[kPlaneSlots];
+    __shared__ unsigned long long vals[kPlaneSlots];
+    const int tid = (int)threadIdx.x;
+    const int col_blocks = a.w / kPlaneCols;
+    const int x0 = (int)(blockIdx.x % (uint32_t)col_blocks) * kPlaneCols;
+    const int y0 = (int)(blockIdx.x / (uint32_t)col_blocks) * kPlaneRows;
+    const int W = a.w, H = a.h;
+    for (int i = tid; i < kPlaneSlots; i += kPlaneCols) {
+        keys[i] = 0xFFFFFFFFu;
+        vals[i] = 0ull;
+    }
+    // rows y0 - 1 .. y0 + kPlaneRows (wrapped): 64 16-byte words a row
+    constexpr int kRowsHeld = kPlaneRows + 2;
+    constexpr int kQuads = kRowsHeld * (kPlaneCols / 4);
+    constexpr int kLoads = (kQuads + kPlaneCols - 1) / kPlaneCols;
+    uint4 v[kLoads];
+#pragma unroll
+    for (int k = 0; k < kLoads; ++k) {
+        const int q = k * kPlaneCols + tid;
+        const int r = q < kQuads ? q / (kPlaneCols / 4) : kRowsHeld - 1;
+        const int c4 = q % (kPlaneCols / 4);
+        int gy = y0 - 1 + r;
+        gy = gy < 0 ? gy + H : gy >= H ? gy - H : gy;
+        v[k] = *reinterpret_cast<const uint4 *>(a.labels + (size_t)gy * (size_t)W + (size_t)(x0 + 4 * c4));
+    }
+    // the halo columns: lane 2 r + side
+    uint32_t hv = 0;
+    const bool h_live = tid < 2 * kRowsHeld;
+    if (h_live) {
+        const int r = tid >> 1;
+        int gy = y0 - 1 + r;
+        gy = gy < 0 ? gy + H : gy >= H ? gy - H : gy;
+        const int gx = (tid & 1) ? (x0 + kPlaneCols == W ? 0 : x0 + kPlaneCols) : (x0 == 0 ? W - 1 : x0 - 1);
+        hv = a.labels[(size_t)gy * (size_t)W + (size_t)gx];
+    }
+#pragma unroll
+    for (int k = 0; k < kLoads; ++k) {
+        const int q = k * kPlaneCols + tid;
+        if (q < kQuads)
+            *reinterpret_cast<uint4 *>(&rt[(q / (kPlaneCols / 4)) * kPlanePitch + 4 + 4 * (q % (kPlaneCols / 4))]) =
+                v[k];
+    }
+    if (h_live) rt[(tid >> 1) * kPlanePitch + ((tid & 1) ? 4 + kPlaneCols : 3)] = hv;
+    __syncthreads();
+
+    auto insert = [&](uint32_t l, int64_t d) {
+        uint32_t h = ((((l >> 4) * 0x9E3779B1u) >> 24) & ~15u) | (l & 15u);
+        for (int probe = 0; probe < kPlaneSlots / 16; ++probe) {
+            h &= (uint32_t)(kPlaneSlots - 1);
+            const uint32_t k = atomicCAS(&keys[h], 0xFFFFFFFFu, l);
+            if (k == 0xFFFFFFFFu || k == l) {
+                atomicAdd(&vals[h], (unsigned long long)d);
+                return;
+            }
+            h += 16;
+        }
+        atomicAdd(reinterpret_cast<unsigned long long *>(a.state + l), (unsigned long long)d);
+    };
+    // a run's volume and bonds ride in one word, volume * 2^16 + bonds: a
+    // wavefront's group sums stay below 64 * kPlaneRows sites and 8 times
+    // as many bonds, inside 16 bits each
+    static_assert(64 * kPlaneRows * 8 < 65536, "a group's bond sum fits 16 bits");
+    auto flush = [&](bool active, uint32_t word, uint32_t acc) {
+        if (__ballot(active) == 0ull) return;
+        uint32_t sum;
+        if (wave_group_sum(active, word, acc, &sum)) insert(label_id(word), pack_vs(sum >> 16, sum & 0xFFFFu));
+    };
+
+    // the 3 x 3 neighbourhood of this lane's site, rows j - 1 .. j + 1; a
+    // run ends where the word changes (a label's word carries its type)
+    const int c = 4 + tid;
+    uint32_t n00 = rt[c - 1], n01 = rt[c], n02 = rt[c + 1];
+    uint32_t n10 = rt[kPlanePitch + c - 1], n11 = rt[kPlanePitch + c], n12 = rt[kPlanePitch + c + 1];
+    uint32_t run = n11;
+    uint32_t acc = 0;
+#pragma unroll 4
+    for (int j = 1; j <= kPlaneRows; ++j) {
+        const uint32_t *below = &rt[(j + 1) * kPlanePitch + c];
+        const uint32_t n20 = below[-1], n21 = below[0], n22 = below[1];
+        const uint32_t word = n11;
+        uint32_t like = (n01 == word) + (n10 == word) + (n12 == word) + (n21 == word);
+        if constexpr (NOFF == 8) like += (n00 == word) + (n02 == word) + (n20 == word) + (n22 == word);
+        const bool ends = word != run;
+        flush(ends, run, acc);
+        if (ends) {
+            run = word;
+            acc = 0;
+        }
+        acc += 65536u + (uint32_t)NOFF - like;
+        n00 = n10; n01 = n11; n02 = n12;
+        n10 = n20; n11 = n21; n12 = n22;
+    }
+    flush(true, run, acc);
+    __syncthreads();
+    for (int i = tid; i < kPlaneSlots; i += kPlaneCols) {
+        const uint32_t l = keys[i];
+        if (l != 0xFFFFFFFFu) atomicAdd(reinterpret_cast<unsigned long long *>(a.state + l), vals[i]);
+    }
+}
+
 static void recount_gated(const CellArgs &a, size_t n_sites, int moments_only, int gate, hipStream_t s) {
+    if (!moments_only && gate < 0 && recount_plane_applies(a)) {
+        const uint32_t blocks = (uint32_t)(a.w / kPlaneCols) * (uint32_t)(a.h / kPlaneRows);
+        if (a.n_off == 8)
+            hipLaunchKernelGGL(recount_plane_kernel<8>, dim3(blocks), dim3(kPlaneCols), 0, s, a);
+        else
+            hipLaunchKernelGGL(recount_plane_kernel<4>, dim3(blocks), dim3(kPlaneCols), 0, s, a);
+        return;
+    }
     hipLaunchKernelGGL(recount_kernel, dim3(recount_blocks(a.w, a.h, a.d)), dim3(kRecountCols), 0, s, a, n_sites,
                        moments_only, gate);
 }
