@@ -47,6 +47,34 @@ def test_lattice_and_bookkeeping_match_the_emulation(engine, oracle_lib, name):
     gpu.close()
 
 
+@pytest.mark.parametrize("order,w,h,scatter", [(1, 512, 256, False), (2, 256, 512, False), (2, 512, 256, True)])
+def test_the_plane_recount_matches_the_emulation(engine, oracle_lib, order, w, h, scatter):
+    # a plane whose records are volume and surface alone is recounted in
+    # blocks of 16 rows by 256 columns: cells of side 7 cut by the blocks'
+    # edges, a medium margin, both planar neighbourhoods, and (scattered
+    # sites) more labels in a block than its table holds
+    from glazier_amd import HipSimulation
+    from tests.cases import ref_model, tiled
+
+    m = ref_model(w, h, seed=41, order=order)
+    labels, types = tiled(m, side=7)
+    if scatter:
+        labels = np.random.default_rng(41).permutation(labels)
+    print(f"[parity] plane recount {w}x{h}, order {order}, scattered {scatter}", flush=True)
+    gpu = HipSimulation(m, labels, types)
+    emu = oracle.EmuSimulation(m, labels, types)
+    assert np.array_equal(gpu.cell_state()[1:], emu.cell_state()[1:])
+    assert np.array_equal(gpu.volumes(), emu.volumes())
+    assert np.array_equal(gpu.surfaces(), emu.surfaces())
+    gpu.step(3)
+    emu.step(3)
+    assert int((gpu.labels() != emu.labels()).sum()) == 0
+    assert np.array_equal(gpu.cell_state()[1:], emu.cell_state()[1:])
+    assert gpu.counters() == emu.counters()
+    gpu.close()
+    emu.close()
+
+
 def test_a_second_call_continues_the_same_run(engine, oracle_lib):
     from glazier_amd import HipSimulation
 
