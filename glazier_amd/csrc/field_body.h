@@ -25,6 +25,9 @@
 namespace ghip {
 
 // One workgroup's tile (block `b` of the launch over this lattice and species).
+// A lane takes whole columns (x, y) of the tile and walks them along z, so a
+// site's LDS index is its column's plus a multiple of the plane (no division
+// per site), and the z neighbours of the walk slide through registers.
 template <bool D3>
 __device__ __forceinline__ void field_body(const FieldArgs &a, const int b) {
     extern __shared__ float fl[];
@@ -32,35 +35,50 @@ __device__ __forceinline__ void field_body(const FieldArgs &a, const int b) {
     const int K = a.steps;
     const int hz = D3 ? K : 0;
     const int LX = IX + 2 * K, LY = IY + 2 * K, LZ = IZ + 2 * hz;
-    const int total = LX * LY * LZ;
+    const int plane = LX * LY;
+    const int total = plane * LZ;
     float *u = fl;
     float *v = fl + total;
     const int tx = b % a.tiles_x, ty = (b / a.tiles_x) % a.tiles_y, tz = D3 ? b / (a.tiles_x * a.tiles_y) : 0;
     const int x0 = tx * IX - K, y0 = ty * IY - K, z0 = D3 ? tz * IZ - hz : 0;
+    const size_t wh = (size_t)a.w * (size_t)a.h;
 
-    for (int q = threadIdx.x; q < total; q += blockDim.x) {
-        int lx = q % LX, ly = (q / LX) % LY, lz = q / (LX * LY);
-        int gx = wrapc(x0 + lx, a.w), gy = wrapc(y0 + ly, a.h), gz = D3 ? wrapc(z0 + lz, a.d) : 0;
-        u[q] = a.in[((size_t)gz * a.h + gy) * (size_t)a.w + gx];
+    for (int c = threadIdx.x; c < plane; c += blockDim.x) {
+        const int ly = c / LX, lx = c - ly * LX;
+        const float *col = a.in + (size_t)wrapc(y0 + ly, a.h) * (size_t)a.w + (size_t)wrapc(x0 + lx, a.w);
+        int gz = D3 ? wrapc(z0, a.d) : 0;  // stepped with the walk, wrapping at the lattice's depth
+#pragma unroll 4
+        for (int lz = 0; lz < LZ; ++lz) {
+            u[lz * plane + c] = col[(size_t)gz * wh];
+            if (D3 && ++gz == a.d) gz = 0;
+        }
     }
     __syncthreads();
 
     const float faces = D3 ? 6.0f : 4.0f;
     for (int s = 1; s <= K; ++s) {
         // valid after s substeps: [s, L - s) on each blocked axis
-        for (int q = threadIdx.x; q < total; q += blockDim.x) {
-            int lx = q % LX, ly = (q / LX) % LY, lz = q / (LX * LY);
+        const int zlo = D3 ? s : 0, zhi = D3 ? LZ - s : 1;
+        for (int c = threadIdx.x; c < plane; c += blockDim.x) {
+            const int ly = c / LX, lx = c - ly * LX;
             if (lx < s || lx >= LX - s || ly < s || ly >= LY - s) continue;
-            if (D3 && (lz < s || lz >= LZ - s)) continue;
+            int q = zlo * plane + c;
             float here = u[q];
-            float sum = u[q + 1] + u[q - 1];
-            sum = sum + u[q + LX];
-            sum = sum + u[q - LX];
-            if (D3) {
-                sum = sum + u[q + LX * LY];
-                sum = sum + u[q - LX * LY];
+            float below = D3 ? u[q - plane] : 0.0f;
+            for (int lz = zlo; lz < zhi; ++lz, q += plane) {
+                float sum = u[q + 1] + u[q - 1];
+                sum = sum + u[q + LX];
+                sum = sum + u[q - LX];
+                float above = 0.0f;
+                if (D3) {
+                    above = u[q + plane];
+                    sum = sum + above;
+                    sum = sum + below;
+                }
+                v[q] = (here + a.dcoef * (sum - faces * here)) - a.kcoef * here;
+                below = here;
+                here = above;
             }
-            v[q] = (here + a.dcoef * (sum - faces * here)) - a.kcoef * here;
         }
         __syncthreads();
         float *t = u;
@@ -68,23 +86,27 @@ __device__ __forceinline__ void field_body(const FieldArgs &a, const int b) {
         v = t;
     }
 
-    const int interior = IX * IY * IZ;
-    for (int q = threadIdx.x; q < interior; q += blockDim.x) {
-        int ix = q & (IX - 1), iy = (q >> a.ix_log2) & (IY - 1), iz = q >> (a.ix_log2 + a.iy_log2);
-        int gx = tx * IX + ix, gy = ty * IY + iy, gz = D3 ? tz * IZ + iz : 0;
-        if (gx >= a.w || gy >= a.h || (D3 && gz >= a.d)) continue;
-        size_t site = ((size_t)gz * a.h + gy) * (size_t)a.w + gx;
-        float val = u[((iz + hz) * LY + iy + K) * LX + ix + K];
-        if (a.exchange) {
-            uint32_t word = a.labels[site];
-            if (label_id(word) != 0) {
-                uint32_t t = label_type(word);
-                float w2 = val + a.tables->sec[t][a.species];
-                w2 = w2 - a.tables->up[t][a.species] * w2;
-                val = w2 < 0.0f ? 0.0f : w2;
+    const int interior = IX * IY;
+    for (int c = threadIdx.x; c < interior; c += blockDim.x) {
+        const int ix = c & (IX - 1), iy = c >> a.ix_log2;
+        const int gx = tx * IX + ix, gy = ty * IY + iy;
+        if (gx >= a.w || gy >= a.h) continue;
+        for (int iz = 0; iz < IZ; ++iz) {
+            const int gz = D3 ? tz * IZ + iz : 0;
+            if (D3 && gz >= a.d) break;
+            const size_t site = (size_t)gz * wh + (size_t)gy * a.w + gx;
+            float val = u[((iz + hz) * LY + iy + K) * LX + ix + K];
+            if (a.exchange) {
+                uint32_t word = a.labels[site];
+                if (label_id(word) != 0) {
+                    uint32_t t = label_type(word);
+                    float w2 = val + a.tables->sec[t][a.species];
+                    w2 = w2 - a.tables->up[t][a.species] * w2;
+                    val = w2 < 0.0f ? 0.0f : w2;
+                }
             }
+            a.out[site] = val;
         }
-        a.out[site] = val;
     }
 }
 
