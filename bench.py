@@ -556,10 +556,11 @@ def main() -> int:
                     help="skip the rocprofv3 counter runs that measure the pass's HBM bytes")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="only run a few passes of one lattice (the counter runs use this)")
-    ap.add_argument("--probe", choices=["config2", "moving4096"], default=None,
+    ap.add_argument("--probe", choices=["config2", "moving4096", "ref256", "ref512"], default=None,
                     help="only step one workload and print its step time, for a profiler run: "
-                         "config2 (512^2 infection sheet, seed 7, 200 steps) or moving4096 "
-                         "(the reference's benchmark model at 4096^2, 25 steps)")
+                         "config2 (512^2 infection sheet, seed 7, 200 steps), moving4096 "
+                         "(the reference's benchmark model at 4096^2, 25 steps), ref256 or ref512 "
+                         "(that model at 256^2 or 512^2, 100 steps)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy a MIB-mebibyte buffer 5 times with torch, so a counter "
                          "run can scale FETCH_SIZE/WRITE_SIZE by a copy of known size")
@@ -587,8 +588,10 @@ def main() -> int:
     if args.probe:
         from glazier_amd.workloads import infection_sheet, reference_model
 
-        built, steps = ((infection_sheet(512, 7), 200) if args.probe == "config2"
-                        else (reference_model(4096, 1), 25))
+        built, steps = {"config2": lambda: (infection_sheet(512, 7), 200),
+                        "moving4096": lambda: (reference_model(4096, 1), 25),
+                        "ref256": lambda: (reference_model(256, 1), 100),
+                        "ref512": lambda: (reference_model(512, 1), 100)}[args.probe]()
         sim = HipSimulation(*built)
         sim.step(1)
         torch.cuda.synchronize()
@@ -596,8 +599,14 @@ def main() -> int:
         sim.step(steps)
         torch.cuda.synchronize()
         el = time.perf_counter() - t0
+        # the same steps again with HIP events around every sweep launch
+        sim.set_timing(True)
+        sim.step(steps)
+        sim.set_timing(False)
+        ev_ms, ev_launches = sim.kernel_time("sweep")
         print(json.dumps({"probe": args.probe, "steps": steps, "us_per_step": el / steps * 1e6,
-                          "n_labels": sim.n_labels()}))
+                          "n_labels": sim.n_labels(), "sweep_launches": ev_launches,
+                          "sweep_event_us_per_step": ev_ms * 1e3 / steps}))
         sim.close()
         return 0
     if args.traffic_probe:

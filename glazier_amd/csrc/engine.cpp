@@ -293,6 +293,52 @@ ghip_status run_resident(ghip_sim *s, uint64_t steps) {
     return GHIP_OK;
 }
 
+// Steps a mid-size plain lattice with one resident workgroup per tile, at
+// most kResidentChunk steps per launch. The launch leaves the records in the
+// rotation a streaming pass expects (the last pass's changes unfolded), and
+// the arrival epoch advances by the barriers it passed.
+ghip_status run_coop(ghip_sim *s, uint64_t steps) {
+    const PlainParams pp = plain_params(s);
+    while (steps > 0) {
+        const uint32_t take = (uint32_t)std::min<uint64_t>(steps, kResidentChunk);
+        const uint32_t barriers = plain2d_coop_barriers(take);
+        if (s->coop_epoch > 0x7FFFFFFFu - barriers) {
+            // restart the epochs before the wrap-safe comparison could fail
+            HIP_TRY(launch_fill_u32(s->coop_sync, kCoopErrWord, 0u, s->stream));
+            s->coop_epoch = 0;
+        }
+        PlainCoopArgs r{};
+        r.labels = s->labels[s->lab_cur];
+        r.a_start = s->A[s->pass % 2];
+        r.a_end = s->A[s->pass % 2];
+        for (int i = 0; i < 3; ++i) r.d[i] = s->Dl[i];
+        r.n_labels = s->n_labels_dev;
+        r.accepted = s->accepted;
+        r.arrive = s->coop_sync;
+        r.err = s->coop_sync + kCoopErrWord;
+        r.seed = s->m.seed;
+        r.mcs0 = s->mcs;
+        r.pass0 = s->pass;
+        r.replica = s->m.replica;
+        r.steps = take;
+        r.epoch0 = s->coop_epoch;
+        r.w_log2 = ilog2(s->W);
+        r.h = s->H;
+        r.tx_log2 = s->coop_tx_log2;
+        r.ty_log2 = s->coop_ty_log2;
+        TimedLaunch t{};
+        HIP_TRY(s->begin_timed(0, &t));
+        HIP_TRY(launch_sweep_plain2d_coop(r, pp, s->coop_blocks, s->stream));
+        HIP_TRY(s->end_timed(t));
+        s->coop_epoch += barriers;
+        s->pass += 4 * (uint64_t)take;
+        s->mcs += take;
+        s->attempted += (uint64_t)s->N * take;
+        steps -= take;
+    }
+    return GHIP_OK;
+}
+
 ghip_status run_sweep_pass(ghip_sim *s, int colour) {
     SweepArgs a = s->sweep_template();
     const uint64_t p = s->pass;
@@ -631,7 +677,9 @@ ghip_status ghip_create_nematic(const ghip_model *model, const uint32_t *labels,
     cap = std::min<uint64_t>(cap, kIdMask - 1);
     s->cap = (uint32_t)cap;
     s->resident = s->plain_physics && m.n_species == 0 && plain2d_resident_fits(s->W, s->H, s->cap);
-    s->quiet_skip = s->plain_fast && !s->resident && !s->tracks_moments && !s->has_pop && !s->has_mot &&
+    s->coop = !s->resident && s->plain_physics && m.n_species == 0 &&
+              plain2d_coop_shape(s->W, s->H, s->cap, &s->coop_tx_log2, &s->coop_ty_log2, &s->coop_blocks);
+    s->quiet_skip = s->plain_fast && !s->resident && !s->coop && !s->tracks_moments && !s->has_pop && !s->has_mot &&
                     !s->has_nem && m.n_species == 0;
     for (uint32_t k = 0; k < m.n_species; ++k) {
         s->substeps[k] = substeps_for(m, k);
@@ -696,6 +744,10 @@ ghip_status ghip_create_nematic(const ghip_model *model, const uint32_t *labels,
     CREATE_TRY(hipMalloc(&s->n_labels_dev, 4));
     CREATE_TRY(hipMalloc(&s->tables, sizeof(DeviceTables)));
     CREATE_TRY(hipMalloc(&s->accepted, kAcceptSlots * 8));
+    if (s->coop) {
+        CREATE_TRY(hipMalloc(&s->coop_sync, kCoopSyncWords * 4));
+        CREATE_TRY(hipMemset(s->coop_sync, 0, kCoopSyncWords * 4));
+    }
 
     // uploads: packed label words, per-label type words, fields, activity
     std::vector<uint32_t> packed(N);
@@ -757,8 +809,13 @@ ghip_status ghip_step(ghip_sim *s, uint64_t steps) {
         if (!s->span_stop) HIP_TRY(hipEventCreate(&s->span_stop));
         HIP_TRY(hipEventRecord(s->span_start, s->stream));
     }
+    const bool coop_run = s->coop && !s->traj && steps > 0;
     if (s->resident && !s->traj) {
         ghip_status st = run_resident(s, steps);
+        if (st != GHIP_OK) return st;
+        steps = 0;
+    } else if (coop_run) {
+        ghip_status st = run_coop(s, steps);
         if (st != GHIP_OK) return st;
         steps = 0;
     }
@@ -800,6 +857,14 @@ ghip_status ghip_step(ghip_sim *s, uint64_t steps) {
         HIP_TRY(hipEventElapsedTime(&elapsed, s->span_start, s->span_stop));
         s->ms[kStepFamily] += elapsed;
         s->launches[kStepFamily] += s->pass - pass0;
+    }
+    if (coop_run) {
+        uint32_t bad = 0;
+        HIP_TRY(hipMemcpy(&bad, s->coop_sync + kCoopErrWord, 4, hipMemcpyDeviceToHost));
+        if (bad) {
+            HIP_TRY(hipMemset(s->coop_sync + kCoopErrWord, 0, 4));
+            return fail(GHIP_DEVICE, "a grid barrier of the resident pass gave up waiting; the lattice is not valid");
+        }
     }
     if (s->has_pop) {
         // word 2 of either set outlives the per-step clears: set by any

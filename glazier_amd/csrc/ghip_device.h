@@ -142,6 +142,33 @@ struct PlainResidentArgs {
     uint32_t capacity;            // label records the lattice may name
 };
 
+// A mid-size plain lattice stepped whole by one resident workgroup per tile
+// (sweep_plain.hip): `steps` Monte Carlo steps from pass `pass0`, passes
+// separated by a grid barrier instead of launches. Records follow the
+// streaming passes' rotation through a and d[pass % 3], so the launch leaves
+// the state the next streaming pass expects: a_end holds the records as the
+// last pass priced them, d[(end + 2) % 3] that pass's changes, d[end % 3]
+// cleared.
+constexpr int kCoopMaxBlocks = 64;                   // one wavefront polls every arrival word
+constexpr int kCoopArriveStride = 32;                // words between arrival words (one line each)
+constexpr int kCoopErrWord = kCoopMaxBlocks * kCoopArriveStride;
+constexpr int kCoopSyncWords = kCoopErrWord + 32;
+struct PlainCoopArgs {
+    uint32_t *labels;             // w x h packed words, current when the launch starts and ends
+    const int64_t *a_start;
+    int64_t *a_end;               // the same buffer as a_start (passes come in fours)
+    int64_t *d[3];                // the pass delta buffers, by pass number mod 3
+    const uint32_t *n_labels;
+    unsigned long long *accepted; // kAcceptSlots counters
+    uint32_t *arrive;             // one epoch word per workgroup, kCoopArriveStride words apart
+    uint32_t *err;                // set when a barrier's wait gave up
+    uint64_t seed, mcs0, pass0;
+    uint32_t replica, steps;
+    uint32_t epoch0;              // the arrival epoch the previous launch ended on
+    int w_log2, h;
+    int tx_log2, ty_log2;         // tile sides
+};
+
 // One launch of the field step: `steps` substeps of one species.
 struct FieldArgs {
     const float *in;        // one species, n sites
@@ -211,6 +238,14 @@ void plain2d_tile(int w, int h, int *px, int *py);
 bool plain2d_resident_fits(int w, int h, uint32_t capacity);
 hipError_t launch_sweep_plain2d_resident(const PlainResidentArgs &r, const PlainParams &p,
                                          hipStream_t stream);
+// The cooperative resident pass's tile for a w x h lattice with `capacity`
+// label records: false when it does not apply. *blocks receives the number
+// of workgroups (tiles) of its launch.
+bool plain2d_coop_shape(int w, int h, uint32_t capacity, int *tx_log2, int *ty_log2, int *blocks);
+// Barriers one launch of `steps` steps passes (the arrival epochs it uses).
+inline uint32_t plain2d_coop_barriers(uint32_t steps) { return 4u * steps - 1u; }
+hipError_t launch_sweep_plain2d_coop(const PlainCoopArgs &r, const PlainParams &p, int blocks,
+                                     hipStream_t stream);
 hipError_t launch_field(const FieldArgs &a, bool d3, int blocks, hipStream_t stream);
 size_t field_lds_bytes(const FieldArgs &a, bool d3);
 // The plane field body's interior tile; launch_field_planes (and the batch's

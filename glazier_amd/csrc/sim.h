@@ -91,6 +91,12 @@ struct ghip_sim {
     bool plain_physics = false;  // the medium, one cell type, integer contact/volume/surface, Moore plane
     bool plain_fast = false;     // the plain-monolayer pass (sweep_plain.hip) applies
     bool resident = false;       // the whole lattice steps inside one workgroup
+    // the whole lattice steps in one resident workgroup per tile, passes
+    // separated by grid barriers (sweep_plain.hip)
+    bool coop = false;
+    int coop_tx_log2 = 0, coop_ty_log2 = 0, coop_blocks = 0;
+    uint32_t *coop_sync = nullptr;  // kCoopSyncWords: arrival words (a line each), then the error word
+    uint32_t coop_epoch = 0;        // the arrival epoch the last launch ended on
     // a step is its plain streaming passes and nothing else, so the passes
     // can tell from pass_flags when the last two accepted nothing anywhere
     bool quiet_skip = false;
@@ -119,7 +125,7 @@ struct ghip_sim {
         void *bufs[] = {labels[0], labels[1], act[0], act[1], field_base[0], field_base[1], A[0],
                         A[1], Dl[0], Dl[1], Dl[2], anchor, ltype, scratch, daughter, plan, geo,
                         flags, type_copy, anchor_copy, n_labels_dev, tables, accepted, traj, len_before,
-                        nematic, pass_flags};
+                        nematic, pass_flags, coop_sync};
         for (void *p : bufs)
             if (p) (void)hipFree(p);
         for (auto *list : {&pending, &spare})

@@ -558,6 +558,7 @@ __global__ __launch_bounds__(NT) void sweep_plain2d_kernel(SweepArgs a, PlainPar
 constexpr int kResThreads = 1024;
 constexpr int kResSites = 16384;   // lattice words held in LDS (64 KiB)
 constexpr int kResLabels = 2048;   // per-label records held in LDS, twice 16 KiB
+constexpr int kResMaxSteps = 64;   // steps per launch
 
 __global__ __launch_bounds__(kResThreads) void sweep_plain2d_resident_kernel(PlainResidentArgs r, PlainParams p) {
     __shared__ __attribute__((aligned(16))) uint32_t lat[kResSites];
@@ -566,6 +567,9 @@ __global__ __launch_bounds__(kResThreads) void sweep_plain2d_resident_kernel(Pla
     __shared__ uint16_t queue[kResSites / 4];
     __shared__ uint32_t thr_s[kPThr];
     __shared__ uint32_t qcount;
+    // every pass's colour and keys, derived once per launch
+    __shared__ uint32_t pass_keys[2 * 4 * kResMaxSteps];
+    __shared__ uint8_t pass_colour[4 * kResMaxSteps];
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -582,6 +586,15 @@ __global__ __launch_bounds__(kResThreads) void sweep_plain2d_resident_kernel(Pla
     }
     if (tid < (int)p.n_thr) thr_s[tid] = p.thr[tid];
     if (tid == 0) qcount = 0;
+    if (tid < (int)(4u * r.steps)) {
+        int order[4];
+        colour_order(r.seed, r.replica, r.mcs0 + (uint32_t)tid / 4, 4, order);
+        const int colour = order[tid & 3];
+        const uint32_t key = pass_key(r.seed, r.replica, r.mcs0 + (uint32_t)tid / 4, (uint32_t)colour);
+        pass_keys[2 * tid] = key;
+        pass_keys[2 * tid + 1] = second_key(key);
+        pass_colour[tid] = (uint8_t)colour;
+    }
     __syncthreads();
 
     const int tw = wl - 1;                         // log2 of targets per row
@@ -589,14 +602,11 @@ __global__ __launch_bounds__(kResThreads) void sweep_plain2d_resident_kernel(Pla
     const uint32_t nt_pad = (nt + kResThreads - 1) / kResThreads * kResThreads;
     uint32_t my_acc = 0;
     for (uint32_t step = 0; step < r.steps; ++step) {
-        const uint64_t mcs = r.mcs0 + step;
-        int order[4];
-        colour_order(r.seed, r.replica, mcs, 4, order);
-        for (int c = 0; c < 4; ++c) {
-            const int colour = order[c];
+        for (uint32_t c = 0; c < 4; ++c) {
+            const uint32_t k = 4 * step + c;
+            const int colour = pass_colour[k];
             const int cx = colour & 1, cy = (colour >> 1) & 1;
-            const uint32_t key = pass_key(r.seed, r.replica, mcs, (uint32_t)colour);
-            const uint32_t key2 = second_key(key);
+            const uint32_t key = pass_keys[2 * k], key2 = pass_keys[2 * k + 1];
 
             // phase A: every target's draw; queue the ones whose label would change
             for (uint32_t q = tid; q < nt_pad; q += kResThreads) {
@@ -640,10 +650,12 @@ __global__ __launch_bounds__(kResThreads) void sweep_plain2d_resident_kernel(Pla
                 const uint32_t n3 = lat[y0 | xm], n4 = lat[y0 | xp];
                 const uint32_t n5 = lat[yp | xm], n6 = lat[yp | (uint32_t)x], n7 = lat[yp | xp];
                 const uint32_t old = lat[site];
-                // the picked neighbour, k in ORDER3's planar order
-                const uint32_t k = ph & 7u;
-                const uint32_t nw = k == 0 ? n0 : k == 1 ? n1 : k == 2 ? n2 : k == 3 ? n3
-                                  : k == 4 ? n4 : k == 5 ? n5 : k == 6 ? n6 : n7;
+                // the picked neighbour, k in ORDER3's planar order, read at
+                // its offset (a select chain on k would branch per lane)
+                const uint32_t sh = (ph & 7u) * 2u;
+                const uint32_t nxp = (uint32_t)((x + (int)((0x9224u >> sh) & 3u) - 1) & (W - 1));
+                const uint32_t dy = (0xA940u >> sh) & 3u;
+                const uint32_t nw = lat[(dy == 0 ? ym : dy == 1 ? y0 : yp) | nxp];
                 const uint32_t oid = label_id(old), nid = label_id(nw);
                 const PlainMove mv = plain_move(p, thr_s, key, key2, site, old, nw, n0, n1, n2, n3, n4, n5, n6, n7,
                                                 rec[oid], rec[nid], p.needs_medium != 0);
@@ -685,9 +697,356 @@ bool plain2d_resident_fits(int w, int h, uint32_t capacity) {
 }
 
 hipError_t launch_sweep_plain2d_resident(const PlainResidentArgs &r, const PlainParams &p, hipStream_t stream) {
-    if (!plain2d_resident_fits(1 << r.w_log2, r.h, r.capacity)) return hipErrorInvalidValue;
+    if (r.steps > (uint32_t)kResMaxSteps || !plain2d_resident_fits(1 << r.w_log2, r.h, r.capacity))
+        return hipErrorInvalidValue;
     hipLaunchKernelGGL(sweep_plain2d_resident_kernel, dim3(1), dim3(kResThreads), 0, stream, r, p);
     return hipGetLastError();
+}
+
+// ---- the cooperative resident pass: a mid-size lattice, one resident
+// workgroup per tile
+//
+// Above the single workgroup's reach and up to a few hundred thousand sites
+// a streaming pass is mostly launch latency: a dependent launch of an empty
+// kernel costs about 2.8 us here, and such a pass about 5.4 us. These
+// lattices are instead cut into at most 64 tiles of up to 128 x 128 sites,
+// one 1024-lane workgroup per tile, all resident at once (a cooperative
+// launch), and a pass ends at a grid barrier instead of a kernel boundary.
+// Each workgroup keeps its tile, a copy of every label's record and the
+// acceptance table in LDS for the whole launch. Per pass it refreshes its
+// one-site halo ring from the lattice in memory, folds the previous pass's
+// record changes into its copy, then draws, prices and commits exactly as
+// the other passes do (the records fixed while the pass prices), writing
+// accepted labels through to memory and adding record changes into the
+// pass's delta buffer.
+//
+// What workgroups share inside the launch is written with agent-scope
+// stores and atomics (which go through to the coherent level) and read with
+// agent-scope loads, so no L2 is written back or invalidated at a barrier:
+// every wave waits for the acknowledgements of its own stores, then its
+// workgroup publishes the barrier's epoch in its own arrival word (each on a
+// cache line of its own: 32 writers on one line cost about 0.3 us a
+// barrier), and wave 0 waits until every workgroup's word has reached that
+// epoch. The wait is
+// bounded: a workgroup that gives up sets the error word and leaves, the
+// others then give up at their next barrier, and the host reports it.
+constexpr int kCoopThreads = 1024;
+constexpr int kCoopSide = 128;        // largest tile side
+constexpr int kCoopPitch = kCoopSide + 2;
+constexpr int kCoopLabels = 8192;     // records held in LDS
+constexpr uint32_t kCoopSpin = 1u << 22;
+constexpr int kCoopMaxSteps = 64;     // steps per launch
+constexpr int kCoopTiles = 32;        // preferred tile count
+
+namespace {
+__device__ __forceinline__ uint32_t load_agent(const uint32_t *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ int64_t load_agent(const int64_t *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void store_agent(uint32_t *p, uint32_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void store_agent(int64_t *p, int64_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void add_agent(int64_t *p, int64_t v) {
+    (void)__hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// All workgroups of the launch meet; false when the wait gave up.
+__device__ __forceinline__ bool coop_barrier(uint32_t *arrive, uint32_t nwg, uint32_t epoch, uint32_t *fail_s,
+                                             uint32_t *err) {
+    __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this wave's stores and atomics acknowledged
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        if (threadIdx.x == 0) store_agent(&arrive[blockIdx.x * kCoopArriveStride], epoch);
+        bool met = false;
+        for (uint32_t it = 0; it < kCoopSpin && !met; ++it)
+            met = __all(threadIdx.x >= nwg || (int32_t)(load_agent(&arrive[threadIdx.x * kCoopArriveStride]) - epoch) >= 0);
+        if (!met && threadIdx.x == 0) {
+            *fail_s = 1;
+            atomicAdd(err, 1u);
+        }
+    }
+    __syncthreads();
+    return *fail_s == 0;
+}
+}  // namespace
+
+__global__ __launch_bounds__(kCoopThreads) void sweep_plain2d_coop_kernel(PlainCoopArgs r, PlainParams p) {
+    __shared__ uint32_t lat[kCoopPitch * kCoopPitch];
+    __shared__ int64_t rec[kCoopLabels];
+    __shared__ uint16_t queue[kCoopSide * kCoopSide / 4];
+    __shared__ uint32_t thr_s[kPThr];
+    __shared__ uint32_t qcount, fail_s;
+    // every pass's colour and keys, derived once per launch
+    __shared__ uint32_t pass_keys[2 * 4 * kCoopMaxSteps];
+    __shared__ uint8_t pass_colour[4 * kCoopMaxSteps];
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wl = r.w_log2, txl = r.tx_log2, tyl = r.ty_log2;
+    const int W = 1 << wl, H = r.h, TX = 1 << txl, TY = 1 << tyl, P = TX + 2;
+    const uint32_t nwg = gridDim.x, b = blockIdx.x;
+    const int x0 = (int)(b & ((1u << (wl - txl)) - 1u)) << txl;
+    const int y0 = (int)(b >> (wl - txl)) << tyl;
+    const uint32_t nl = *r.n_labels;
+    if (nl > (uint32_t)kCoopLabels) {
+        // every workgroup leaves here, before any barrier
+        if (b == 0 && tid == 0) atomicAdd(r.err, 1u);
+        return;
+    }
+    // this workgroup's share of the delta buffer a pass clears
+    const uint32_t share = (nl + nwg - 1) / nwg;
+    const uint32_t lo = min(b * share, nl), hi = min(lo + share, nl);
+
+    // every load of a lane issued before the first wait: the tile in 16-byte
+    // quads (at most four a lane), the records (at most eight)
+    constexpr int kQuads = kCoopSide * kCoopSide / 4 / kCoopThreads;
+    constexpr int kRecs = kCoopLabels / kCoopThreads;
+    {
+        const int quads = (TX * TY) >> 2, qrow = TX >> 2;
+        uint4 v[kQuads];
+#pragma unroll
+        for (int j = 0; j < kQuads; ++j) {
+            const int i = tid + j * kCoopThreads;
+            if (i < quads) {
+                const int ly = i / qrow, lq = i - ly * qrow;
+                v[j] = reinterpret_cast<const uint4 *>(r.labels)[(((uint32_t)(y0 + ly) << wl) |
+                                                                  (uint32_t)(x0 + 4 * lq)) >> 2];
+            }
+        }
+        int64_t a[kRecs];
+#pragma unroll
+        for (int j = 0; j < kRecs; ++j) {
+            const uint32_t i = tid + j * kCoopThreads;
+            if (i < nl) a[j] = r.a_start[i];
+        }
+#pragma unroll
+        for (int j = 0; j < kQuads; ++j) {
+            const int i = tid + j * kCoopThreads;
+            if (i < quads) {
+                const int ly = i / qrow, lq = i - ly * qrow;
+                uint32_t *row = &lat[(ly + 1) * P + 1 + 4 * lq];
+                row[0] = v[j].x;
+                row[1] = v[j].y;
+                row[2] = v[j].z;
+                row[3] = v[j].w;
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < kRecs; ++j) {
+            const uint32_t i = tid + j * kCoopThreads;
+            if (i < nl) rec[i] = a[j];
+        }
+    }
+    if (tid < (int)p.n_thr) thr_s[tid] = p.thr[tid];
+    if (tid == 0) fail_s = 0;
+    const uint32_t passes = 4u * r.steps;
+    if (tid < (int)passes) {
+        int order[4];
+        colour_order(r.seed, r.replica, r.mcs0 + (uint32_t)tid / 4, 4, order);
+        const int colour = order[tid & 3];
+        const uint32_t key = pass_key(r.seed, r.replica, r.mcs0 + (uint32_t)tid / 4, (uint32_t)colour);
+        pass_keys[2 * tid] = key;
+        pass_keys[2 * tid + 1] = second_key(key);
+        pass_colour[tid] = (uint8_t)colour;
+    }
+
+    // the halo ring, one site per lane: the rows above and below (corners
+    // included), then the columns left and right
+    const int ring = 2 * (TX + 2) + 2 * TY;
+    int ring_lds = 0;
+    uint32_t ring_site = 0;
+    if (tid < ring) {
+        int lx, ly;
+        if (tid < TX + 2) {
+            lx = tid - 1;
+            ly = -1;
+        } else if (tid < 2 * (TX + 2)) {
+            lx = tid - (TX + 2) - 1;
+            ly = TY;
+        } else if (tid < 2 * (TX + 2) + TY) {
+            lx = -1;
+            ly = tid - 2 * (TX + 2);
+        } else {
+            lx = TX;
+            ly = tid - 2 * (TX + 2) - TY;
+        }
+        int gy = y0 + ly;
+        gy += gy < 0 ? H : 0;
+        gy -= gy >= H ? H : 0;
+        ring_site = ((uint32_t)gy << wl) | ((uint32_t)(x0 + lx) & (uint32_t)(W - 1));
+        ring_lds = (ly + 1) * P + lx + 1;
+    }
+
+    const int tw = txl - 1;                       // log2 of targets per tile row
+    const uint32_t nt = (uint32_t)(TX * TY) >> 2;  // targets per pass, a multiple of 64
+    uint32_t my_acc = 0;
+    uint32_t epoch = r.epoch0;
+    uint32_t rot = (uint32_t)(r.pass0 % 3);  // this pass's delta buffer
+    __syncthreads();                          // the pass table is read from here on
+    for (uint32_t k = 0; k < passes; ++k) {
+        const int colour = pass_colour[k];
+        const int cx = colour & 1, cy = (colour >> 1) & 1;
+        const uint32_t key = pass_keys[2 * k], key2 = pass_keys[2 * k + 1];
+        const int64_t *d_prev = r.d[rot == 0 ? 2 : rot - 1];
+        int64_t *d_acc = r.d[rot];
+        int64_t *d_clear = r.d[rot == 2 ? 0 : rot + 1];
+        rot = rot == 2 ? 0 : rot + 1;
+
+        // the halo as the last pass left it, the records as this pass prices them
+        {
+            const uint32_t halo = tid < ring ? load_agent(&r.labels[ring_site]) : 0u;
+            int64_t dv[kRecs];
+#pragma unroll
+            for (int j = 0; j < kRecs; ++j) {
+                const uint32_t i = tid + j * kCoopThreads;
+                if (i < nl) dv[j] = load_agent(d_prev + i);
+            }
+            for (uint32_t i = lo + tid; i < hi; i += kCoopThreads) store_agent(d_clear + i, (int64_t)0);
+            if (tid < ring) lat[ring_lds] = halo;
+            // LDS adds with no return: no read round trip per record
+#pragma unroll
+            for (int j = 0; j < kRecs; ++j) {
+                const uint32_t i = tid + j * kCoopThreads;
+                if (i < nl) (void)__hip_atomic_fetch_add(&rec[i], dv[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            }
+        }
+        if (tid == 0) qcount = 0;
+        __syncthreads();
+
+        if (nt <= (uint32_t)kCoopThreads) {
+            // at most one target a lane: each lane prices its own target where
+            // its label would change, with no queue
+            if (tid < (int)nt) {
+                const uint32_t q = (uint32_t)tid;
+                const int lx = 2 * (int)(q & ((1u << tw) - 1u)) + cx;
+                const int ly = 2 * (int)(q >> tw) + cy;
+                const uint32_t site = ((uint32_t)(y0 + ly) << wl) | (uint32_t)(x0 + lx);
+                const uint32_t pk = site_pick(key, site) & 7u;
+                const uint32_t sh = pk * 2u;
+                const int tl = (ly + 1) * P + lx + 1;
+                const uint32_t old = lat[tl];
+                const uint32_t nw = lat[tl + (int)((0x9224u >> sh) & 3u) - 1 + ((int)((0xA940u >> sh) & 3u) - 1) * P];
+                if (old != nw) {
+                    const uint32_t n0 = lat[tl - P - 1], n1 = lat[tl - P], n2 = lat[tl - P + 1];
+                    const uint32_t n3 = lat[tl - 1], n4 = lat[tl + 1];
+                    const uint32_t n5 = lat[tl + P - 1], n6 = lat[tl + P], n7 = lat[tl + P + 1];
+                    const uint32_t oid = label_id(old), nid = label_id(nw);
+                    const PlainMove mv = plain_move(p, thr_s, key, key2, site, old, nw, n0, n1, n2, n3, n4, n5, n6,
+                                                    n7, rec[oid], rec[nid], p.needs_medium != 0);
+                    if (mv.acc) {
+                        lat[tl] = nw;
+                        store_agent(&r.labels[site], nw);
+                        if (oid != 0) add_agent(d_acc + oid, pack_vs(-1, mv.ds_old));
+                        if (nid != 0) add_agent(d_acc + nid, pack_vs(1, mv.ds_new));
+                        ++my_acc;
+                    }
+                }
+            }
+        } else {
+            // phase A: every target's draw; queue (target, pick) where the label would change
+            for (uint32_t q = tid; q < nt; q += kCoopThreads) {
+                const int lx = 2 * (int)(q & ((1u << tw) - 1u)) + cx;
+                const int ly = 2 * (int)(q >> tw) + cy;
+                const uint32_t site = ((uint32_t)(y0 + ly) << wl) | (uint32_t)(x0 + lx);
+                const uint32_t pk = site_pick(key, site) & 7u;
+                const uint32_t sh = pk * 2u;
+                const int tl = (ly + 1) * P + lx + 1;
+                const int nb = tl + (int)((0x9224u >> sh) & 3u) - 1 + ((int)((0xA940u >> sh) & 3u) - 1) * P;
+                const bool moves = lat[tl] != lat[nb];
+                const unsigned long long m = __ballot(moves);
+                if (m == 0) continue;
+                uint32_t base = 0;
+                if (lane == 0) base = atomicAdd(&qcount, (uint32_t)__builtin_popcountll(m));
+                base = __builtin_amdgcn_readfirstlane(base);
+                if (moves) {
+                    const uint32_t pos = base + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                                                          __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+                    queue[pos] = (uint16_t)(q | pk << 12);
+                }
+            }
+            __syncthreads();
+
+            // phase B: price the queued targets against the pass-start records
+            const uint32_t nq = qcount;
+            for (uint32_t qi = tid; qi < nq; qi += kCoopThreads) {
+                const uint32_t e = queue[qi];
+                const uint32_t q = e & 0xFFFu, pk = e >> 12;
+                const int lx = 2 * (int)(q & ((1u << tw) - 1u)) + cx;
+                const int ly = 2 * (int)(q >> tw) + cy;
+                const uint32_t site = ((uint32_t)(y0 + ly) << wl) | (uint32_t)(x0 + lx);
+                const int tl = (ly + 1) * P + lx + 1;
+                const uint32_t n0 = lat[tl - P - 1], n1 = lat[tl - P], n2 = lat[tl - P + 1];
+                const uint32_t n3 = lat[tl - 1], n4 = lat[tl + 1];
+                const uint32_t n5 = lat[tl + P - 1], n6 = lat[tl + P], n7 = lat[tl + P + 1];
+                const uint32_t old = lat[tl];
+                // the picked neighbour read at its offset (a select chain on the
+                // pick would branch per lane)
+                const uint32_t sh = pk * 2u;
+                const uint32_t nw = lat[tl + (int)((0x9224u >> sh) & 3u) - 1 + ((int)((0xA940u >> sh) & 3u) - 1) * P];
+                const uint32_t oid = label_id(old), nid = label_id(nw);
+                const PlainMove mv = plain_move(p, thr_s, key, key2, site, old, nw, n0, n1, n2, n3, n4, n5, n6, n7,
+                                                rec[oid], rec[nid], p.needs_medium != 0);
+                if (!mv.acc) continue;
+                lat[tl] = nw;
+                store_agent(&r.labels[site], nw);
+                if (oid != 0) add_agent(d_acc + oid, pack_vs(-1, mv.ds_old));
+                if (nid != 0) add_agent(d_acc + nid, pack_vs(1, mv.ds_new));
+                ++my_acc;
+            }
+        }
+        if (k + 1 == passes) break;
+        if (!coop_barrier(r.arrive, nwg, ++epoch, &fail_s, r.err)) return;
+    }
+
+    // the records as the last pass priced them; its changes stay in its delta buffer
+    for (uint32_t i = lo + tid; i < hi; i += kCoopThreads) r.a_end[i] = rec[i];
+    uint32_t t = my_acc;
+    for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off, 64);
+    if (lane == 0 && t) atomicAdd(r.accepted + ((b * 16u + (uint32_t)(tid >> 6)) & (kAcceptSlots - 1)),
+                                  (unsigned long long)t);
+}
+
+bool plain2d_coop_shape(int w, int h, uint32_t capacity, int *tx_log2, int *ty_log2, int *blocks) {
+    const bool pow2 = w >= 32 && (w & (w - 1)) == 0;
+    if (!pow2 || h < 32 || capacity > (uint32_t)kCoopLabels || (size_t)w * (size_t)h <= (size_t)kResSites)
+        return false;
+    // the shape whose tile count is nearest kCoopTiles, squarer tiles first
+    int best = -1, best_tx = 0, best_ty = 0;
+    for (int txl = 7; txl >= 5; --txl)
+        for (int tyl = 7; tyl >= 5; --tyl) {
+            const int tx = 1 << txl, ty = 1 << tyl;
+            if (tx > w || ty > h || h % ty != 0) continue;
+            const long n = (long)(w / tx) * (long)(h / ty);
+            if (n > kCoopMaxBlocks) continue;
+            const int score = (int)(n > kCoopTiles ? n - kCoopTiles : kCoopTiles - n) * 8 + (txl > tyl ? txl - tyl : tyl - txl);
+            if (best < 0 || score < best) {
+                best = score;
+                best_tx = txl;
+                best_ty = tyl;
+            }
+        }
+    if (best < 0) return false;
+    *tx_log2 = best_tx;
+    *ty_log2 = best_ty;
+    *blocks = (w >> best_tx) * (h >> best_ty);
+    return true;
+}
+
+hipError_t launch_sweep_plain2d_coop(const PlainCoopArgs &r, const PlainParams &p, int blocks, hipStream_t stream) {
+    int txl = 0, tyl = 0, n = 0;
+    if (r.steps == 0 || r.steps > (uint32_t)kCoopMaxSteps || !plain2d_coop_shape(1 << r.w_log2, r.h, 0, &txl, &tyl, &n) || txl != r.tx_log2 ||
+        tyl != r.ty_log2 || n != blocks)
+        return hipErrorInvalidValue;
+    PlainCoopArgs a = r;
+    PlainParams pp = p;
+    void *args[] = {&a, &pp};
+    return hipLaunchCooperativeKernel(reinterpret_cast<const void *>(sweep_plain2d_coop_kernel), dim3(blocks),
+                                      dim3(kCoopThreads), args, 0, stream);
 }
 
 bool plain2d_supported(uint32_t n_thr) { return n_thr <= (uint32_t)kPThr; }

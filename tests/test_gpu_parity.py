@@ -125,6 +125,70 @@ def test_a_resident_lattice_matches_across_launch_and_call_boundaries(engine, or
     gpu.close()
 
 
+def coop_case(name):
+    from tests.cases import ref_model, tiled
+
+    if name == "ref256":
+        m = ref_model(256, 256, seed=21)
+        return (m, *tiled(m))
+    if name == "ref512":
+        # 4096 cells: 4097 records in every workgroup's copy
+        m = ref_model(512, 512, seed=22)
+        return (m, *tiled(m))
+    if name == "medium512x256":
+        m = ref_model(512, 256, seed=23)
+        l, t = tiled(m, side=8, nx=50, ny=28)
+        t[0] = 0
+        return m, l, t
+    if name in ("scattered256", "scattered512"):
+        # every cell's sites scattered: most targets would change label
+        side = int(name[-3:])
+        m = ref_model(side, side, seed=24)
+        l, t = tiled(m)
+        return m, np.random.default_rng(24).permutation(l), t
+    if name == "wide1024x32":
+        # one row of tiles: the rows above and below a tile are its own
+        m = ref_model(1024, 32, seed=25)
+        return (m, *tiled(m))
+    if name == "tall32x1024":
+        # one column of tiles: the columns either side are its own
+        m = ref_model(32, 1024, seed=26)
+        return (m, *tiled(m))
+    raise KeyError(name)
+
+
+@pytest.mark.parametrize("name,chunks", [("ref256", (70, 3, 1)), ("ref512", (5,)), ("medium512x256", (2, 9)),
+                                         ("scattered256", (4,)), ("scattered512", (2,)), ("wide1024x32", (3, 2)),
+                                         ("tall32x1024", (3,))])
+def test_a_tiled_resident_lattice_matches_across_barriers_launches_and_calls(engine, oracle_lib, name, chunks):
+    # plain lattices above the one-workgroup size step with one resident
+    # workgroup per tile, passes separated by grid barriers, at most 64
+    # steps per launch: runs that cross barriers, launches and calls match
+    # the emulation, with tiles of at most one target a lane (priced where
+    # drawn) and of more (queued)
+    from glazier_amd import HipSimulation
+
+    m, labels, types = coop_case(name)
+    steps = sum(chunks)
+    print(f"[parity] tiled resident {name}: {m.width}x{m.height}, chunks {chunks}", flush=True)
+    gpu = HipSimulation(m, labels, types)
+    gpu.set_timing(True)
+    for c in chunks:
+        gpu.step(c)
+    _, launches = gpu.kernel_time("sweep")
+    assert launches == sum((c + 63) // 64 for c in chunks), "the tiled resident pass did not run"
+    emu = oracle.EmuSimulation(m, labels, types)
+    emu.step(steps)
+    assert int((gpu.labels() != emu.labels()).sum()) == 0
+    assert np.array_equal(gpu.cell_state()[1:], emu.cell_state()[1:])
+    assert np.array_equal(gpu.volumes(), emu.volumes())
+    assert np.array_equal(gpu.surfaces(), emu.surfaces())
+    assert gpu.counters() == emu.counters()
+    assert gpu.mcs == steps
+    gpu.close()
+    emu.close()
+
+
 def test_span_timing_counts_the_passes_of_each_step_call(engine):
     from glazier_amd import HipSimulation
     from glazier_amd.engine import GhipError
