@@ -143,7 +143,8 @@ struct PlainResidentArgs {
 };
 
 // A mid-size plain lattice stepped whole by one resident workgroup per tile
-// (sweep_plain.hip): `steps` Monte Carlo steps from pass `pass0`, passes
+// (sweep_plain.hip, a cooperative launch): `steps` Monte Carlo steps (at
+// most 64) from pass `pass0`, passes
 // separated by a grid barrier instead of launches. Records follow the
 // streaming passes' rotation through a and d[pass % 3], so the launch leaves
 // the state the next streaming pass expects: a_end holds the records as the
@@ -238,7 +239,7 @@ void plain2d_tile(int w, int h, int *px, int *py);
 bool plain2d_resident_fits(int w, int h, uint32_t capacity);
 hipError_t launch_sweep_plain2d_resident(const PlainResidentArgs &r, const PlainParams &p,
                                          hipStream_t stream);
-// The cooperative resident pass's tile for a w x h lattice with `capacity`
+// The tiled resident pass's tile for a w x h lattice with `capacity`
 // label records: false when it does not apply. *blocks receives the number
 // of workgroups (tiles) of its launch.
 bool plain2d_coop_shape(int w, int h, uint32_t capacity, int *tx_log2, int *ty_log2, int *blocks);

@@ -703,21 +703,25 @@ hipError_t launch_sweep_plain2d_resident(const PlainResidentArgs &r, const Plain
     return hipGetLastError();
 }
 
-// ---- the cooperative resident pass: a mid-size lattice, one resident
-// workgroup per tile
+// ---- the tiled resident pass: a mid-size lattice, one resident workgroup
+// per tile
 //
-// Above the single workgroup's reach and up to a few hundred thousand sites
-// a streaming pass is mostly launch latency: a dependent launch of an empty
-// kernel costs about 2.8 us here, and such a pass about 5.4 us. These
-// lattices are instead cut into at most 64 tiles of up to 128 x 128 sites,
-// one 1024-lane workgroup per tile, all resident at once (a cooperative
-// launch), and a pass ends at a grid barrier instead of a kernel boundary.
-// Each workgroup keeps its tile, a copy of every label's record and the
-// acceptance table in LDS for the whole launch. Per pass it refreshes its
-// one-site halo ring from the lattice in memory, folds the previous pass's
-// record changes into its copy, then draws, prices and commits exactly as
-// the other passes do (the records fixed while the pass prices), writing
-// accepted labels through to memory and adding record changes into the
+// Above the single workgroup's reach a streaming pass on a lattice of up to
+// a million sites is mostly latency: each pass is a dependent launch around
+// a chain of memory round trips.
+// These lattices are instead cut into at most 64 tiles of up to 128 x 128
+// sites (the count nearest 32, which measured fastest), one 1024-lane
+// workgroup per tile, all resident at once (a cooperative launch), and a
+// pass ends at a grid barrier instead of a kernel boundary, up to 64 steps
+// per launch. Each workgroup keeps its tile, a copy of every label's record,
+// the acceptance table and every pass's colour and keys in LDS for the
+// whole launch. Per pass it refreshes its one-site halo ring from the
+// lattice in memory and folds the previous pass's record changes into its
+// copy (all of a lane's loads in flight together), then draws, prices and
+// commits exactly as the other passes do (the records fixed while the pass
+// prices): a tile of at most one target a lane prices each target in the
+// lane that drew it, a larger one queues its changing targets first. It
+// writes accepted labels through to memory and adds record changes into the
 // pass's delta buffer.
 //
 // What workgroups share inside the launch is written with agent-scope
@@ -727,9 +731,9 @@ hipError_t launch_sweep_plain2d_resident(const PlainResidentArgs &r, const Plain
 // workgroup publishes the barrier's epoch in its own arrival word (each on a
 // cache line of its own: 32 writers on one line cost about 0.3 us a
 // barrier), and wave 0 waits until every workgroup's word has reached that
-// epoch. The wait is
-// bounded: a workgroup that gives up sets the error word and leaves, the
-// others then give up at their next barrier, and the host reports it.
+// epoch. The wait is bounded: a workgroup that gives up sets the error word
+// and leaves, the others then give up at their next barrier, and the host
+// reports it.
 constexpr int kCoopThreads = 1024;
 constexpr int kCoopSide = 128;        // largest tile side
 constexpr int kCoopPitch = kCoopSide + 2;
