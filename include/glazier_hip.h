@@ -133,7 +133,10 @@ ghip_status ghip_set_nematic_field(ghip_sim *sim, const double *q, size_t n);
 ghip_status ghip_destroy(ghip_sim *sim);
 
 /* Replaces GpuSimulation::step (glazier-cuda/src/lib.rs:1055-1144): runs
- * `steps` Monte Carlo steps and returns once they are done. */
+ * `steps` Monte Carlo steps and returns once they are done. GHIP_DEVICE
+ * when the label capacity ran out at the id limit, or when a grid barrier
+ * of the tiled resident pass (plain planes above 128^2 sites) gave up
+ * waiting; the lattice is then not valid. */
 ghip_status ghip_step(ghip_sim *sim, uint64_t steps);
 
 /* Replaces the public field GpuSimulation::mcs (lib.rs:805). */
