@@ -17,6 +17,7 @@ size_t sweep_lds_bytes(const SweepArgs &a, bool d3) {
     size_t words = (size_t)a.n_types * a.n_types + 4 * (size_t)a.n_types + a.n_thr + 64 +
                    (size_t)LX * LY * LZ;
     if (a.flags & kMotility) words += (size_t)LX * LY * LZ;
+    words += 1 + (size_t)TX * TY * TZ;  // the queue of targets whose label would change
     return words * 4;
 }
 

@@ -6,8 +6,12 @@
 //  * a workgroup owns a tile of 2TX x 2TY (x 2TZ) sites; it stages the tile's
 //    labels plus a Moore halo in LDS with all lanes' loads in flight, so every target
 //    reads its neighbourhood from LDS instead of strided global gathers;
-//  * one lane per target: TX targets along x per row, so a 64-lane wavefront
-//    covers one target row of a 128-site tile row;
+//  * one lane per target draws it: TX targets along x per row, so a 64-lane
+//    wavefront covers one target row of a 128-site tile row; the targets
+//    whose label would change are queued in LDS, and the gathers, pricing
+//    and commit run on the queue's dense prefix of lanes, one target each
+//    (in a sheet about one target in five is queued, so most wavefronts
+//    skip the pricing whole instead of running it for a few lanes);
 //  * each target takes ONE 32-bit counter hash (pick from the low bits,
 //    uniform from the top 24), so no RNG state is read or written;
 //  * per-label volume/surface/moments are read from the pass-start snapshot
@@ -94,6 +98,8 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &a, const int blk) {
     int *sOff = reinterpret_cast<int *>(sThr + a.n_thr); // [0,32) LDS deltas, [32,64) packed xyz
     uint32_t *sLab = reinterpret_cast<uint32_t *>(sOff + 64);
     float *sAct = reinterpret_cast<float *>(sLab + LX * LY * LZ);
+    // the queue of targets whose label would change: a count, then lane ids
+    uint32_t *sQ = reinterpret_cast<uint32_t *>(sAct) + ((a.flags & kMotility) ? LX * LY * LZ : 0);
 
     // ---- every load of the prologue goes out before the first wait on one:
     // the labels (and activity) of the tile with its halo, as one flat range
@@ -165,6 +171,7 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &a, const int blk) {
         sOff[tid] = a.off[tid][0] + a.off[tid][1] * LX + a.off[tid][2] * LX * LY;
         sOff[32 + tid] = (a.off[tid][0] + 1) | ((a.off[tid][1] + 1) << 2) | ((a.off[tid][2] + 1) << 4);
     }
+    if (tid == 0) sQ[0] = 0;
     stage_store(0, lv0, av0);
     // a tile larger than kStage loads per lane stages the rest in turn
     for (int base = kStage * nthreads; base < staged; base += kStage * nthreads) {
@@ -176,47 +183,81 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &a, const int blk) {
     const int wave = tid >> 6, lane = tid & 63, nwaves = nthreads >> 6;
     __syncthreads();
 
-    // ---- this lane's target.
-    const int i = tid & (TX - 1);
-    const int j = (tid >> a.tx_log2) & (TY - 1);
-    const int k = D3 ? (tid >> (a.tx_log2 + a.ty_log2)) : 0;
-    const int gx = x0 + 2 * i + a.cx, gy = y0 + 2 * j + a.cy, gz = D3 ? z0 + 2 * k + a.cz : 0;
-    bool accepted = false;
-    int tl_keep = 0;
-    uint32_t nw_keep = 0;
-    float act_keep = -1.0f;
-    const bool active = tid < TX * TY * TZ && gx < a.w && gy < a.h && (!D3 || gz < a.d);
-    if (active) {
+    // ---- target t of the tile (lane t's in the drawing): its coordinates,
+    // its site in the tile, its draws and the picked source
+    struct Target {
+        int gx, gy, gz, tl, sl, pdx, pdy, pdz;
+        uint32_t site, r24;
+        bool active;
+    };
+    auto target = [&](int t) {
+        Target g;
+        const int i = t & (TX - 1);
+        const int j = (t >> a.tx_log2) & (TY - 1);
+        const int k = D3 ? (t >> (a.tx_log2 + a.ty_log2)) : 0;
+        g.gx = x0 + 2 * i + a.cx;
+        g.gy = y0 + 2 * j + a.cy;
+        g.gz = D3 ? z0 + 2 * k + a.cz : 0;
+        g.active = t < TX * TY * TZ && g.gx < a.w && g.gy < a.h && (!D3 || g.gz < a.d);
         const int lx = halo + 2 * i + a.cx, ly = halo + 2 * j + a.cy, lz = D3 ? halo + 2 * k + a.cz : 0;
-        const int tl = (lz * LY + ly) * LX + lx;
-        const uint32_t site = (uint32_t)(((size_t)gz * a.h + gy) * (size_t)a.w + gx);
-        const uint32_t ph = site_pick(a.key, site);
+        g.tl = (lz * LY + ly) * LX + lx;
+        g.site = (uint32_t)(((size_t)g.gz * a.h + g.gy) * (size_t)a.w + g.gx);
+        const uint32_t ph = site_pick(a.key, g.site);
         const uint32_t h = site_draw(a.key2, ph);
         int pick;
-        uint32_t r24;
-        int pdx, pdy, pdz;
         if (MOORE2D) {
             pick = (int)(ph & 7u);
-            r24 = h >> 8;
+            g.r24 = h >> 8;
             int q = pick + (pick >= 4 ? 1 : 0);
-            pdy = q / 3 - 1;
-            pdx = q - 3 * (pdy + 1) - 1;
-            pdz = 0;
+            g.pdy = q / 3 - 1;
+            g.pdx = q - 3 * (g.pdy + 1) - 1;
+            g.pdz = 0;
         } else {
             const int n = a.n_off;
             if ((n & (n - 1)) == 0) {
                 pick = (int)(ph & (uint32_t)(n - 1));
-                r24 = h >> 8;
+                g.r24 = h >> 8;
             } else {
                 pick = (int)(((uint64_t)h * (uint64_t)n) >> 32);
-                r24 = second_uniform(h);
+                g.r24 = second_uniform(h);
             }
             const int packed = sOff[32 + pick];
-            pdx = (packed & 3) - 1;
-            pdy = ((packed >> 2) & 3) - 1;
-            pdz = ((packed >> 4) & 3) - 1;
+            g.pdx = (packed & 3) - 1;
+            g.pdy = ((packed >> 2) & 3) - 1;
+            g.pdz = ((packed >> 4) & 3) - 1;
         }
-        const int sl = tl + pdx + pdy * LX + pdz * LX * LY;
+        g.sl = g.tl + g.pdx + g.pdy * LX + g.pdz * LX * LY;
+        return g;
+    };
+
+    // ---- every lane draws its own target and queues it when its label
+    // would change; the pricing below then runs on a dense prefix of the
+    // lanes (in a sheet about one target in five is queued), so a wavefront
+    // of queued targets replaces several mostly idle ones. What a target
+    // decides and commits does not depend on the lane that prices it.
+    {
+        const Target g = target(tid);
+        const bool cand = g.active && sLab[g.tl] != sLab[g.sl];
+        const unsigned long long m = __ballot(cand);
+        uint32_t base = 0;
+        if (lane == 0 && m) base = atomicAdd(&sQ[0], (uint32_t)__builtin_popcountll(m));
+        base = __builtin_amdgcn_readfirstlane(base);
+        if (cand)
+            sQ[1 + base + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u))] =
+                (uint32_t)tid;
+    }
+    __syncthreads();
+    const uint32_t nq = sQ[0];
+
+    bool accepted = false;
+    int tl_keep = 0;
+    uint32_t nw_keep = 0;
+    float act_keep = -1.0f;
+    if (tid < (int)nq) {
+        const Target g = target((int)sQ[1 + tid]);
+        const int gx = g.gx, gy = g.gy, gz = g.gz;
+        const int tl = g.tl, sl = g.sl, pdx = g.pdx, pdy = g.pdy, pdz = g.pdz;
+        const uint32_t site = g.site, r24 = g.r24;
         const uint32_t old = sLab[tl];
         const uint32_t nw = sLab[sl];
         const uint32_t oid = label_id(old), nid = label_id(nw);
