@@ -322,6 +322,7 @@ ghip_status run_coop(ghip_sim *s, uint64_t steps) {
         r.replica = s->m.replica;
         r.steps = take;
         r.epoch0 = s->coop_epoch;
+        r.patience = s->coop_patience;
         r.w_log2 = ilog2(s->W);
         r.h = s->H;
         r.tx_log2 = s->coop_tx_log2;
@@ -1057,6 +1058,12 @@ ghip_status ghip_cell_state(const ghip_sim *s, int64_t *out, size_t n, uint32_t 
         out[words * l + 1] = unpack_surf(c[0]);
         for (uint32_t q = 2; q < words; ++q) out[words * l + q] = c[q - 1];
     }
+    return GHIP_OK;
+}
+
+ghip_status ghip_set_barrier_patience(ghip_sim *s, uint32_t polls) {
+    if (!s) return fail(GHIP_BAD_ARG, "null handle");
+    s->coop_patience = polls;
     return GHIP_OK;
 }
 

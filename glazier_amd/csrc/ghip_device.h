@@ -154,6 +154,7 @@ constexpr int kCoopMaxBlocks = 64;                   // one wavefront polls ever
 constexpr int kCoopArriveStride = 32;                // words between arrival words (one line each)
 constexpr int kCoopErrWord = kCoopMaxBlocks * kCoopArriveStride;
 constexpr int kCoopSyncWords = kCoopErrWord + 32;
+constexpr uint32_t kCoopPatience = 1u << 22;  // a barrier's polls before it gives up, by default
 struct PlainCoopArgs {
     uint32_t *labels;             // w x h packed words, current when the launch starts and ends
     const int64_t *a_start;
@@ -166,6 +167,7 @@ struct PlainCoopArgs {
     uint64_t seed, mcs0, pass0;
     uint32_t replica, steps;
     uint32_t epoch0;              // the arrival epoch the previous launch ended on
+    uint32_t patience;            // polls a barrier makes before it gives up (0: gives up at once)
     int w_log2, h;
     int tx_log2, ty_log2;         // tile sides
 };

@@ -97,6 +97,7 @@ struct ghip_sim {
     int coop_tx_log2 = 0, coop_ty_log2 = 0, coop_blocks = 0;
     uint32_t *coop_sync = nullptr;  // kCoopSyncWords: arrival words (a line each), then the error word
     uint32_t coop_epoch = 0;        // the arrival epoch the last launch ended on
+    uint32_t coop_patience = ghip::kCoopPatience;
     // a step is its plain streaming passes and nothing else, so the passes
     // can tell from pass_flags when the last two accepted nothing anywhere
     bool quiet_skip = false;

@@ -738,7 +738,6 @@ constexpr int kCoopThreads = 1024;
 constexpr int kCoopSide = 128;        // largest tile side
 constexpr int kCoopPitch = kCoopSide + 2;
 constexpr int kCoopLabels = 8192;     // records held in LDS
-constexpr uint32_t kCoopSpin = 1u << 22;
 constexpr int kCoopMaxSteps = 64;     // steps per launch
 constexpr int kCoopTiles = 32;        // preferred tile count
 
@@ -760,14 +759,14 @@ __device__ __forceinline__ void add_agent(int64_t *p, int64_t v) {
 }
 
 // All workgroups of the launch meet; false when the wait gave up.
-__device__ __forceinline__ bool coop_barrier(uint32_t *arrive, uint32_t nwg, uint32_t epoch, uint32_t *fail_s,
-                                             uint32_t *err) {
+__device__ __forceinline__ bool coop_barrier(uint32_t *arrive, uint32_t nwg, uint32_t epoch, uint32_t patience,
+                                             uint32_t *fail_s, uint32_t *err) {
     __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this wave's stores and atomics acknowledged
     __syncthreads();
     if (threadIdx.x < 64) {
         if (threadIdx.x == 0) store_agent(&arrive[blockIdx.x * kCoopArriveStride], epoch);
         bool met = false;
-        for (uint32_t it = 0; it < kCoopSpin && !met; ++it)
+        for (uint32_t it = 0; it < patience && !met; ++it)
             met = __all(threadIdx.x >= nwg || (int32_t)(load_agent(&arrive[threadIdx.x * kCoopArriveStride]) - epoch) >= 0);
         if (!met && threadIdx.x == 0) {
             *fail_s = 1;
@@ -1004,7 +1003,7 @@ __global__ __launch_bounds__(kCoopThreads) void sweep_plain2d_coop_kernel(PlainC
             }
         }
         if (k + 1 == passes) break;
-        if (!coop_barrier(r.arrive, nwg, ++epoch, &fail_s, r.err)) return;
+        if (!coop_barrier(r.arrive, nwg, ++epoch, r.patience, &fail_s, r.err)) return;
     }
 
     // the records as the last pass priced them; its changes stay in its delta buffer

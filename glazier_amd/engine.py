@@ -82,6 +82,7 @@ def library() -> ctypes.CDLL:
     lib.ghip_counters.argtypes = [sim, P(ctypes.c_uint64), P(ctypes.c_uint64)]
     lib.ghip_cell_state.argtypes = [sim, P(ctypes.c_int64), ctypes.c_size_t, P(ctypes.c_uint32)]
     lib.ghip_set_timing.argtypes = [sim, ctypes.c_int]
+    lib.ghip_set_barrier_patience.argtypes = [sim, ctypes.c_uint32]
     lib.ghip_kernel_time.argtypes = [sim, ctypes.c_char_p, P(ctypes.c_double), P(ctypes.c_uint64)]
     lib.ghip_batch_create.argtypes = [P(ctypes.c_void_p), ctypes.c_size_t, P(ctypes.c_void_p)]
     lib.ghip_batch_destroy.argtypes = [ctypes.c_void_p]
@@ -239,6 +240,11 @@ class HipSimulation:
         each step call, summed under family "step"; False/0: off."""
         mode = int(on) if not isinstance(on, bool) else (1 if on else 0)
         _check(library().ghip_set_timing(self._h, mode))
+
+    def set_barrier_patience(self, polls: int) -> None:
+        """Polls a grid barrier of the tiled resident pass makes before it
+        gives up (the step then raises); 0 gives up at the first barrier."""
+        _check(library().ghip_set_barrier_patience(self._h, int(polls)))
 
     def kernel_time(self, family: str):
         ms, n = ctypes.c_double(), ctypes.c_uint64()

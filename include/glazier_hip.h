@@ -194,6 +194,13 @@ ghip_status ghip_cell_state(const ghip_sim *sim, int64_t *out, size_t n,
  * ghip_kernel_time returns the summed milliseconds and the launch count of
  * one family: "sweep", "fields", "moments", "population" or "step". */
 ghip_status ghip_set_timing(ghip_sim *sim, int on);
+
+/* No counterpart in the reference. The polls each grid barrier of the tiled
+ * resident pass (plain planes above 128^2 sites) makes before it gives up,
+ * sets the error ghip_step reports and lets every workgroup of the launch
+ * leave (default 2^22, seconds). 0 gives up at the first barrier, so a test
+ * of that failure path runs with every kernel exiting normally. */
+ghip_status ghip_set_barrier_patience(ghip_sim *sim, uint32_t polls);
 ghip_status ghip_kernel_time(const ghip_sim *sim, const char *family, double *total_ms,
                              uint64_t *launches);
 

@@ -189,6 +189,31 @@ def test_a_tiled_resident_lattice_matches_across_barriers_launches_and_calls(eng
     emu.close()
 
 
+def test_a_tiled_barrier_that_gives_up_is_reported_and_the_device_stays_usable(engine, oracle_lib):
+    # the barrier's wait bounded to no polls from the host: every workgroup of
+    # the launch gives up at the first barrier and leaves, so every kernel
+    # ends normally; the step reports it, and a new lattice afterwards steps
+    # as the emulation does
+    from glazier_amd import GhipError, HipSimulation
+
+    m, labels, types = coop_case("ref256")
+    print("[parity] tiled resident barrier with no patience", flush=True)
+    gpu = HipSimulation(m, labels, types)
+    gpu.set_barrier_patience(0)
+    with pytest.raises(GhipError) as err:
+        gpu.step(2)
+    assert "gave up" in str(err.value)
+    gpu.close()
+    again = HipSimulation(m, labels, types)
+    again.step(3)
+    emu = oracle.EmuSimulation(m, labels, types)
+    emu.step(3)
+    assert int((again.labels() != emu.labels()).sum()) == 0
+    assert again.counters() == emu.counters()
+    again.close()
+    emu.close()
+
+
 def test_span_timing_counts_the_passes_of_each_step_call(engine):
     from glazier_amd import HipSimulation
     from glazier_amd.engine import GhipError
