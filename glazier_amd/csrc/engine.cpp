@@ -680,8 +680,11 @@ ghip_status ghip_create_nematic(const ghip_model *model, const uint32_t *labels,
     s->resident = s->plain_physics && m.n_species == 0 && plain2d_resident_fits(s->W, s->H, s->cap);
     s->coop = !s->resident && s->plain_physics && m.n_species == 0 &&
               plain2d_coop_shape(s->W, s->H, s->cap, &s->coop_tx_log2, &s->coop_ty_log2, &s->coop_blocks);
-    s->quiet_skip = s->plain_fast && !s->resident && !s->coop && !s->tracks_moments && !s->has_pop && !s->has_mot &&
-                    !s->has_nem && m.n_species == 0;
+    auto quiet_skip = [&] {
+        return s->plain_fast && !s->resident && !s->coop && !s->tracks_moments && !s->has_pop && !s->has_mot &&
+               !s->has_nem && m.n_species == 0;
+    };
+    s->quiet_skip = quiet_skip();
     for (uint32_t k = 0; k < m.n_species; ++k) {
         s->substeps[k] = substeps_for(m, k);
         s->fd[k] = (float)(m.species[k].diffusion / (double)s->substeps[k]);
@@ -704,6 +707,12 @@ ghip_status ghip_create_nematic(const ghip_model *model, const uint32_t *labels,
         int dev = 0;
         CREATE_TRY(hipGetDevice(&dev));
         CREATE_TRY(hipDeviceGetAttribute(&s->cus, hipDeviceAttributeMultiprocessorCount, dev));
+    }
+    if (s->coop && !plain2d_coop_resident(s->coop_blocks, s->cus)) {
+        // the device cannot hold every tile's workgroup at once (fewer CUs
+        // than tiles): streaming passes instead
+        s->coop = false;
+        s->quiet_skip = quiet_skip();
     }
     CREATE_TRY(hipMalloc(&s->labels[0], N * 4));
     if (s->has_mot) CREATE_TRY(hipMalloc(&s->labels[1], N * 4));

@@ -245,6 +245,9 @@ hipError_t launch_sweep_plain2d_resident(const PlainResidentArgs &r, const Plain
 // label records: false when it does not apply. *blocks receives the number
 // of workgroups (tiles) of its launch.
 bool plain2d_coop_shape(int w, int h, uint32_t capacity, int *tx_log2, int *ty_log2, int *blocks);
+// Whether `blocks` workgroups of the tiled resident pass fit the device at
+// once (its cooperative launch needs every one resident).
+bool plain2d_coop_resident(int blocks, int cus);
 // Barriers one launch of `steps` steps passes (the arrival epochs it uses).
 inline uint32_t plain2d_coop_barriers(uint32_t steps) { return 4u * steps - 1u; }
 hipError_t launch_sweep_plain2d_coop(const PlainCoopArgs &r, const PlainParams &p, int blocks,

@@ -1040,6 +1040,14 @@ bool plain2d_coop_shape(int w, int h, uint32_t capacity, int *tx_log2, int *ty_l
     return true;
 }
 
+bool plain2d_coop_resident(int blocks, int cus) {
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, sweep_plain2d_coop_kernel, kCoopThreads, 0) !=
+        hipSuccess)
+        return false;
+    return (long)per_cu * (long)cus >= (long)blocks;
+}
+
 hipError_t launch_sweep_plain2d_coop(const PlainCoopArgs &r, const PlainParams &p, int blocks, hipStream_t stream) {
     int txl = 0, tyl = 0, n = 0;
     if (r.steps == 0 || r.steps > (uint32_t)kCoopMaxSteps || !plain2d_coop_shape(1 << r.w_log2, r.h, 0, &txl, &tyl, &n) || txl != r.tx_log2 ||
