@@ -114,6 +114,7 @@ def library() -> ctypes.CDLL:
         "emu_set_field": ([v, u32, P(f64)], None),
         "emu_activity_f32": ([v, P(f32)], None),
         "emu_delta_energy": ([v, sz, sz], f64),
+        "emu_length": ([P(i64), i64, ctypes.c_int], f64),
         "emu_fields_step": ([v], None),
         "emu_one_piece": ([u32, ctypes.c_int], ctypes.c_int),
         "emu_activity_mean": ([v, sz, u32], f32),

@@ -354,10 +354,11 @@ static int needs_recentre(const emu_sim *e) {
 
 /* ---------------------------------------------------------------- geometry */
 
-/* Largest eigenvalue of the scaled covariance C = n S2 - S1 S1^T (exact
- * integers), in f64. The plane uses the closed form; a volume runs three
- * sweeps of cyclic Jacobi rotations (within about 1e-10 of the largest
- * eigenvalue on integer covariances, the tests check it against numpy). */
+/* Largest eigenvalue and its axis of the scaled covariance C = n S2 - S1 S1^T
+ * (exact integers), in f64, for the division plan. The plane uses the closed
+ * form; a volume runs three sweeps of cyclic Jacobi rotations (within about
+ * 1e-10 of the largest eigenvalue on integer covariances). Lengths in a
+ * volume take the closed form largest_eigen3 below. */
 static double cov_largest(const int64_t *m, int64_t n, int three_d, double axis[3]) {
     double cxx = (double)(n * m[3] - m[0] * m[0]);
     double cyy = (double)(n * m[4] - m[1] * m[1]);
@@ -429,7 +430,6 @@ static double cov_largest(const int64_t *m, int64_t n, int three_d, double axis[
     return a[best][best] > 0.0 ? a[best][best] : 0.0;
 }
 
-/* Major axis length 4 sqrt(lambda_max(cov)) from integer sums of n sites. */
 /* In-plane anisotropy of the scaled covariance (the reference's
  * Moments::anisotropy, moments.rs:131-141, is scale-free): (C_xx - C_yy)/tr
  * and 2 C_xy/tr with tr = C_xx + C_yy, zero below two sites or at tr = 0
@@ -447,12 +447,70 @@ static void aniso(const int64_t *m, int64_t n, double *a0, double *a1) {
     *a1 = 2.0 * (double)cxy / (double)tr;
 }
 
+// h(u) = cos(2/3 acos u) for u in [0, 1], analytic there, as a degree-14
+// polynomial in t = 2u - 1 (a Chebyshev fit at 200 001 nodes written in
+// powers of t; largest error 3.9e-14). cos(acos(r)/3) = h(sqrt((1 + r)/2))
+// for r in [-1, 1].
+static double third_angle_cos(double u) {
+    const double t = 2.0 * u - 1.0;
+    double h = -1.7072390069472367e-09;
+    h = h * t + 5.577076476677568e-09;
+    h = h * t + -1.24175244399134e-08;
+    h = h * t + 4.321603925338742e-08;
+    h = h * t + -1.6036631021280584e-07;
+    h = h * t + 5.662416587052154e-07;
+    h = h * t + -2.035473767371784e-06;
+    h = h * t + 7.540205193337502e-06;
+    h = h * t + -2.892003887918587e-05;
+    h = h * t + 0.00011642563899772694;
+    h = h * t + -0.0005041246781468507;
+    h = h * t + 0.0024663527976783333;
+    h = h * t + -0.015509188437110516;
+    h = h * t + 0.24740906632332305;
+    h = h * t + 0.7660444431189831;
+    return h;
+}
+
+// Largest eigenvalue of the symmetric 3 x 3 matrix with diagonal (a00, a11,
+// a22) and off-diagonal (a01, a02, a12), by the trigonometric closed form
+// (O. K. Smith, Comm. ACM 4 (1961) 168): with q = tr/3, p^2 = |A - qI|^2/6
+// and r = det((A - qI)/p)/2 in [-1, 1], it is q + 2p cos(acos(r)/3). Plain
+// multiplies and adds, one division and two square roots, written the same
+// in the emulation.
+static double largest_eigen3(double a00, double a11, double a22, double a01, double a02, double a12) {
+    const double q = (a00 + a11 + a22) / 3.0;
+    const double d0 = a00 - q, d1 = a11 - q, d2 = a22 - q;
+    const double p2 = d0 * d0 + d1 * d1 + d2 * d2 + 2.0 * (a01 * a01 + a02 * a02 + a12 * a12);
+    if (!(p2 > 0.0)) return q;
+    const double p = sqrt(p2 / 6.0);
+    const double ip = 1.0 / p;
+    const double b00 = d0 * ip, b11 = d1 * ip, b22 = d2 * ip;
+    const double b01 = a01 * ip, b02 = a02 * ip, b12 = a12 * ip;
+    const double det = b00 * (b11 * b22 - b12 * b12) - b01 * (b01 * b22 - b12 * b02) + b02 * (b01 * b12 - b11 * b02);
+    double r = 0.5 * det;
+    r = r < -1.0 ? -1.0 : (r > 1.0 ? 1.0 : r);
+    return q + 2.0 * p * third_angle_cos(sqrt(0.5 * (1.0 + r)));
+}
+
+/* Major axis 4 sqrt(lambda_max(cov)) from integer sums of n sites: the
+ * planar closed form in a plane, the 3 x 3 closed form above in a volume
+ * (the division axis keeps the Jacobi rotations of cov_largest). */
 static double length_of(const int64_t *m, int64_t n, int three_d) {
     if (n < 2) return 0.0;
-    double lam = cov_largest(m, n, three_d, NULL);
+    double lam;
+    if (three_d) {
+        lam = largest_eigen3((double)(n * m[3] - m[0] * m[0]), (double)(n * m[4] - m[1] * m[1]),
+                             (double)(n * m[5] - m[2] * m[2]), (double)(n * m[6] - m[0] * m[1]),
+                             (double)(n * m[7] - m[0] * m[2]), (double)(n * m[8] - m[1] * m[2]));
+    } else {
+        lam = cov_largest(m, n, 0, NULL);
+    }
     if (lam < 0.0) lam = 0.0;
     return 4.0 * sqrt(lam) / (double)n;
 }
+
+/* The length above, for the tests: m holds the nine moment sums. */
+double emu_length(const int64_t *m, int64_t n, int three_d) { return length_of(m, n, three_d); }
 
 /* ---------------------------------------------------------------- creation */
 

@@ -113,6 +113,9 @@ void emu_field_f32(const emu_sim *e, uint32_t species, float *out);
 void emu_set_field(emu_sim *e, uint32_t species, const double *values);
 void emu_activity_f32(const emu_sim *e, float *out);
 double emu_delta_energy(const emu_sim *e, size_t target, size_t source);
+/* Major axis length 4 sqrt(lambda_max) of n sites from their nine moment
+ * sums (sx, sy, sz, sxx, syy, szz, sxy, sxz, syz), as the pricing takes it. */
+double emu_length(const int64_t *m, int64_t n, int three_d);
 void emu_fields_step(emu_sim *e);
 int emu_one_piece(uint32_t present, int ring_size);
 

@@ -43,6 +43,8 @@ def engine_host(tmp_path_factory):
     lib.engine_ring_table.restype = None
     lib.engine_pick8.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_long]
     lib.engine_pick8.restype = None
+    lib.engine_length.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_long]
+    lib.engine_length.restype = None
     return lib
 
 
@@ -201,3 +203,59 @@ def test_the_plain_pass_pick_is_the_hash_pick(engine_host):
     lib = oracle.library()
     for i in range(64):
         assert lib.oracle_site_pick(int(key[i]), int(site[i])) & 7 == int(want[i])
+
+
+def _point_sets(rng):
+    """Site clouds (integer coordinates about an anchor) whose covariances
+    cover the eigenvalue cases: random clouds, cubes (three equal
+    eigenvalues), slabs (two equal largest), rods (two equal smallest), the
+    same less a site or two (nearly equal), pairs and collinear sites."""
+    sets = [rng.integers(-12, 13, size=(int(rng.integers(2, 600)), 3)) for _ in range(300)]
+    grid = lambda a, b, c: np.stack(np.meshgrid(np.arange(a), np.arange(b), np.arange(c), indexing="ij"),
+                                    -1).reshape(-1, 3)
+    cube, slab, rod = grid(8, 8, 8), grid(10, 10, 2), grid(2, 2, 20)
+    sets += [cube, cube - 3, cube[1:], cube[2:], slab, slab[1:], rod, rod[1:], grid(5, 7, 11)]
+    sets += [np.array([[0, 0, 0], [1, 0, 0]]), np.array([[0, 0, 0], [0, 0, 0]]),
+             np.array([[0, 0, 0], [3, 4, 5], [6, 8, 10]])]
+    # random ellipsoids of sites, turned by a random rotation
+    for _ in range(60):
+        q, _ = np.linalg.qr(rng.normal(size=(3, 3)))
+        pts = rng.normal(size=(400, 3)) * rng.uniform(0.5, 6.0, size=3) @ q.T
+        sets.append(np.rint(pts).astype(np.int64))
+    return sets
+
+
+def _sums(pts):
+    x, y, z = (pts[:, k].astype(np.int64) for k in range(3))
+    return np.array([x.sum(), y.sum(), z.sum(), (x * x).sum(), (y * y).sum(), (z * z).sum(),
+                     (x * y).sum(), (x * z).sum(), (y * z).sum()], dtype=np.int64)
+
+
+def test_a_volumes_length_is_its_largest_covariance_eigenvalue(oracle_lib):
+    # the closed form the pricing takes in a volume, 4 sqrt(lambda_max)/n of
+    # the scaled covariance, against numpy's eigensolver on the same sites
+    lib = oracle.library()
+    P64 = ctypes.POINTER(ctypes.c_int64)
+    for pts in _point_sets(np.random.default_rng(11)):
+        m = _sums(pts)
+        n = len(pts)
+        got = lib.emu_length(m.ctypes.data_as(P64), n, 1)
+        cov = np.cov(pts.T.astype(np.float64), bias=True)
+        want = 4.0 * np.sqrt(max(np.linalg.eigvalsh(cov)[-1], 0.0))
+        assert abs(got - want) <= 1e-9 * max(want, 1e-3), (n, got, want)
+
+
+def test_the_kernels_length_is_the_emulations_bit_for_bit(engine_host, oracle_lib):
+    # geometry.h built for the host against the emulation's restatement, in
+    # a volume and in a plane
+    lib = oracle.library()
+    P64 = ctypes.POINTER(ctypes.c_int64)
+    sets = _point_sets(np.random.default_rng(12))
+    m = np.stack([_sums(p) for p in sets])
+    counts = np.array([len(p) for p in sets], dtype=np.int64)
+    for three_d in (1, 0):
+        out = np.empty(len(sets))
+        engine_host.engine_length(m.ctypes.data, counts.ctypes.data, three_d, out.ctypes.data, len(sets))
+        want = np.array([lib.emu_length(np.ascontiguousarray(m[i]).ctypes.data_as(P64), int(counts[i]), three_d)
+                         for i in range(len(sets))])
+        assert np.array_equal(out, want), three_d
