@@ -108,15 +108,17 @@ def test_every_occupancy_cap_gives_the_same_lattice(engine, oracle_lib, candidat
 def test_a_lattice_stepped_alone_between_batch_steps(engine, oracle_lib):
     from glazier_amd import HipBatch
 
-    # plain128 steps alone on the resident single-workgroup pass
-    runs = build(["infection128", "motility", "connected3d", "plain128"])
+    # alone, plain128 steps on the resident single-workgroup pass and
+    # surface256 on the tiled resident pass (one workgroup per tile, grid
+    # barriers), between steps of the batch's own passes over the same buffers
+    runs = build(["infection128", "motility", "connected3d", "plain128", "surface256"])
     batch = HipBatch([r[2] for r in runs])
     batch.step(7)
-    runs[1][2].step(2)
-    runs[3][2].step(2)
+    for i in (1, 3, 4):
+        runs[i][2].step(2)
     batch.step(3)
     for i, (name, m, gpu, emu) in enumerate(runs):
-        emu.step(12 if i in (1, 3) else 10)
+        emu.step(12 if i in (1, 3, 4) else 10)
         assert_same(name, m, gpu, emu)
     batch.close()
 
